@@ -1,0 +1,288 @@
+"""birdwing_amd -- MI355X (gfx950) NTT big-integer multiplier.
+
+Python binding of the C ABI in include/birdwing.h (libbirdwing_hip.so, built
+in-tree by `make -C birdwing_amd`).  The library is required: every entry
+point raises if it is missing, there is no CPU fallback.
+
+Host-side mirror of the reference interface (nikitaborisov/birdwing):
+  * multiply(A, B)            -- host_multiply_merge / host_multiply_merge_native
+                                 (src/host_multiply.cpp:27-134): little-endian
+                                 limb arrays in, L_A + L_B limbs out.
+  * Plan                      -- precompute_ntt + upload_ntt_precomputed +
+                                 allocate_ntt_context (src/gpu_ntt.cu:240-440) and
+                                 execute_ntt_multiply (:442-973) on device buffers.
+  * size policy functions     -- src/ntt_limits.cpp.
+  * crt_combine / carry       -- src/crt_gpu.cu, src/carry_prop.cu entry points.
+Device buffers are torch tensors on a HIP device (torch is plumbing for device
+memory and streams only).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = [
+    "lib", "Plan", "multiply", "BWError", "SET_DEFAULT", "SET_U32_REF", "SET_U32_WIDE", "SET_U64_REF",
+    "padded_ntt_size", "multiply_size_supported", "ntt_size_supported", "crt_bound_satisfied",
+    "max_supported_logN", "max_supported_limb_count", "max_limb_count_by_crt", "default_prime_set",
+    "prime_set_info", "crt_combine", "carry128", "carry160", "zero_pad", "zero_pad_u64", "pointwise",
+]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbirdwing_hip.so")
+
+SET_DEFAULT, SET_U32_REF, SET_U32_WIDE, SET_U64_REF, SET_HYBRID = -1, 0, 1, 2, 3
+BW_OK, BW_ERR_ARG, BW_ERR_UNSUPPORTED_SIZE, BW_ERR_HIP, BW_ERR_NOMEM, BW_ERR_STATE = 0, -1, -2, -3, -4, -5
+
+
+class BWError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("birdwing error %d: %s" % (code, msg))
+        self.code = code
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("limb_bits", C.c_int), ("prime_set", C.c_int), ("k", C.c_int),
+                ("primes", C.c_uint64 * 3), ("L_A", C.c_uint64), ("L_B", C.c_uint64),
+                ("N", C.c_uint64), ("N_exec", C.c_uint64), ("npass", C.c_int),
+                ("pass_log", C.c_int * 4), ("nlaunch", C.c_int), ("device_bytes", C.c_uint64)]
+
+
+_vp, _u64, _i, _sz, _cp = C.c_void_p, C.c_uint64, C.c_int, C.c_size_t, C.c_char_p
+# name -> (restype, argtypes); every symbol declared in include/birdwing.h
+SIGNATURES = {
+    "bw_last_error": (_cp, []),
+    "bw_version": (_cp, []),
+    "bw_init": (_i, [_i]),
+    "bw_padded_ntt_size": (_u64, [_u64, _u64]),
+    "bw_default_prime_set": (_i, [_i, _u64]),
+    "bw_prime_set_info": (_i, [_i, C.POINTER(_i), C.POINTER(_u64), C.POINTER(_u64)]),
+    "bw_max_supported_logN": (_i, [_i]),
+    "bw_max_limb_count_by_crt": (_u64, [_i, _i]),
+    "bw_max_supported_limb_count": (_u64, [_i, _i]),
+    "bw_ntt_size_supported": (_i, [_i, _u64, _cp, _sz]),
+    "bw_crt_bound_satisfied": (_i, [_i, _i, _u64, _u64, _cp, _sz]),
+    "bw_multiply_size_supported": (_i, [_i, _i, _u64, _u64, _cp, _sz]),
+    "bw_plan_create": (_i, [C.POINTER(_vp), _i, _u64, _u64, _i]),
+    "bw_plan_create_ex": (_i, [C.POINTER(_vp), _i, _u64, _u64, _i, C.POINTER(_i), _i]),
+    "bw_plan_destroy": (None, [_vp]),
+    "bw_plan_get_info": (_i, [_vp, C.POINTER(PlanInfo)]),
+    "bw_execute": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "bw_execute_crt": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "bw_execute_host": (_i, [_vp, _vp, _vp, _vp]),
+    "bw_plan_status": (_i, [_vp]),
+    "bw_plan_set_timing": (_i, [_vp, _i]),
+    "bw_plan_timing": (_i, [_vp, C.POINTER(C.c_float), _i, C.POINTER(_i)]),
+    "bw_multiply_host": (_i, [_i, _vp, _u64, _vp, _u64, _vp, C.POINTER(C.c_double)]),
+    "bw_forward": (_i, [_vp, _vp, _u64, _vp, _vp]),
+    "bw_inverse": (_i, [_vp, _vp, _vp]),
+    "bw_spectrum_order": (_i, [_vp, C.POINTER(_u64)]),
+    "bw_zero_pad": (_i, [_vp, _vp, _i, _u64, _u64, _vp]),
+    "bw_zero_pad_u64": (_i, [_vp, _vp, _u64, _u64, _u64, _vp]),
+    "bw_pointwise": (_i, [_i, _vp, _vp, _vp, _u64, _u64, _vp]),
+    "bw_crt_combine": (_i, [_i, _vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp]),
+    "bw_carry128": (_i, [_vp, _vp, _u64, _vp, _u64, _i, _vp]),
+    "bw_carry160": (_i, [_vp, _vp, _vp, _u64, _vp, _u64, _vp]),
+}
+
+_LIB = None
+
+
+def lib():
+    """Load libbirdwing_hip.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("birdwing_amd: %s is missing; build it with `make -C %s`" % (LIB_PATH, HERE))
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != BW_OK:
+        raise BWError(rc, lib().bw_last_error().decode())
+    return rc
+
+
+def _dt(bits):
+    return np.uint64 if bits == 64 else np.uint32
+
+
+def _ptr(x):
+    """Device/host address of a torch tensor or numpy array."""
+    if x is None:
+        return None
+    if hasattr(x, "data_ptr"):
+        return C.c_void_p(x.data_ptr())
+    return x.ctypes.data_as(C.c_void_p)
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return C.c_void_p(stream)
+    return C.c_void_p(stream.cuda_stream)
+
+
+# ---- size policy (src/ntt_limits.cpp) --------------------------------------
+def padded_ntt_size(L_A, L_B):
+    return int(lib().bw_padded_ntt_size(L_A, L_B))
+
+
+def default_prime_set(limb_bits, N):
+    return int(lib().bw_default_prime_set(limb_bits, N))
+
+
+def prime_set_info(s):
+    k = C.c_int()
+    P = (C.c_uint64 * 3)()
+    R = (C.c_uint64 * 3)()
+    _check(lib().bw_prime_set_info(s, C.byref(k), P, R))
+    return [int(P[i]) for i in range(k.value)], [int(R[i]) for i in range(k.value)]
+
+
+def max_supported_logN(s):
+    return int(lib().bw_max_supported_logN(s))
+
+
+def max_supported_limb_count(s, in_bits):
+    return int(lib().bw_max_supported_limb_count(s, in_bits))
+
+
+def max_limb_count_by_crt(s, in_bits):
+    return int(lib().bw_max_limb_count_by_crt(s, in_bits))
+
+
+def _why_call(fn, *args):
+    buf = C.create_string_buffer(512)
+    ok = fn(*args, buf, 512)
+    return bool(ok), buf.value.decode()
+
+
+def ntt_size_supported(s, N):
+    return _why_call(lib().bw_ntt_size_supported, s, N)
+
+
+def crt_bound_satisfied(s, in_bits, L_A, L_B):
+    return _why_call(lib().bw_crt_bound_satisfied, s, in_bits, L_A, L_B)
+
+
+def multiply_size_supported(s, in_bits, L_A, L_B):
+    return _why_call(lib().bw_multiply_size_supported, s, in_bits, L_A, L_B)
+
+
+# ---- plans ----------------------------------------------------------------
+class Plan:
+    """A product shape (limb width, L_A, L_B): tables, buffers and launches."""
+
+    def __init__(self, limb_bits, L_A, L_B, prime_set=SET_DEFAULT, passes=None):
+        self._p = C.c_void_p()
+        if passes:
+            arr = (C.c_int * len(passes))(*passes)
+            _check(lib().bw_plan_create_ex(C.byref(self._p), limb_bits, L_A, L_B, prime_set, arr, len(passes)))
+        else:
+            _check(lib().bw_plan_create(C.byref(self._p), limb_bits, L_A, L_B, prime_set))
+        self.limb_bits, self.L_A, self.L_B = limb_bits, L_A, L_B
+        info = PlanInfo()
+        _check(lib().bw_plan_get_info(self._p, C.byref(info)))
+        self.info = info
+        self.N, self.N_exec, self.k = int(info.N), int(info.N_exec), int(info.k)
+        self.primes = [int(info.primes[i]) for i in range(self.k)]
+        self.passes = [int(info.pass_log[i]) for i in range(info.npass)]
+        self.prime_set = int(info.prime_set)
+
+    def close(self):
+        if self._p:
+            lib().bw_plan_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def execute(self, a, b, c, stream=None):
+        """Device tensors: a (L_A limbs), b (L_B limbs) -> c (L_A+L_B limbs)."""
+        _check(lib().bw_execute(self._p, _ptr(a), _ptr(b), _ptr(c), _stream(stream)))
+
+    def execute_crt(self, a, b, c0, c1, c2=None, stream=None):
+        _check(lib().bw_execute_crt(self._p, _ptr(a), _ptr(b), _ptr(c0), _ptr(c1), _ptr(c2), _stream(stream)))
+
+    def execute_host(self, A, B):
+        dt = _dt(self.limb_bits)
+        A = np.ascontiguousarray(A, dtype=dt)
+        B = np.ascontiguousarray(B, dtype=dt)
+        out = np.zeros(self.L_A + self.L_B, dtype=dt)
+        _check(lib().bw_execute_host(self._p, _ptr(A), _ptr(B), _ptr(out)))
+        return out
+
+    def forward(self, raw, L, spec, stream=None):
+        _check(lib().bw_forward(self._p, _ptr(raw), L, _ptr(spec), _stream(stream)))
+
+    def inverse(self, spec, stream=None):
+        _check(lib().bw_inverse(self._p, _ptr(spec), _stream(stream)))
+
+    def spectrum_order(self):
+        order = np.zeros(self.N_exec, dtype=np.uint64)
+        _check(lib().bw_spectrum_order(self._p, order.ctypes.data_as(C.POINTER(C.c_uint64))))
+        return order
+
+    def status(self):
+        _check(lib().bw_plan_status(self._p))
+
+    def set_timing(self, on):
+        _check(lib().bw_plan_set_timing(self._p, 1 if on else 0))
+
+    def timing(self):
+        ms = (C.c_float * 16)()
+        n = C.c_int()
+        _check(lib().bw_plan_timing(self._p, ms, 16, C.byref(n)))
+        return [float(ms[i]) for i in range(n.value)]
+
+
+def multiply(A, B, limb_bits=None):
+    """Host limb arrays (numpy, little-endian) -> product limbs (L_A + L_B)."""
+    if limb_bits is None:
+        limb_bits = 64 if np.asarray(A).dtype == np.uint64 else 32
+    dt = _dt(limb_bits)
+    A = np.ascontiguousarray(A, dtype=dt)
+    B = np.ascontiguousarray(B, dtype=dt)
+    out = np.zeros(len(A) + len(B), dtype=dt)
+    ms = C.c_double()
+    _check(lib().bw_multiply_host(limb_bits, _ptr(A), len(A), _ptr(B), len(B), _ptr(out), C.byref(ms)))
+    return out
+
+
+# ---- stage-level entry points (device tensors) ------------------------------
+def zero_pad(src, dst, L, N, stream=None):
+    wb = dst.element_size()
+    _check(lib().bw_zero_pad(_ptr(src), _ptr(dst), wb, L, N, _stream(stream)))
+
+
+def zero_pad_u64(src, dst, L, N, p, stream=None):
+    _check(lib().bw_zero_pad_u64(_ptr(src), _ptr(dst), L, N, p, _stream(stream)))
+
+
+def pointwise(a, b, c, N, p, stream=None):
+    _check(lib().bw_pointwise(a.element_size(), _ptr(a), _ptr(b), _ptr(c), N, p, _stream(stream)))
+
+
+def crt_combine(prime_set, r0, r1, r2, N, o0, o1, o2=None, stream=None):
+    _check(lib().bw_crt_combine(prime_set, _ptr(r0), _ptr(r1), _ptr(r2), N, _ptr(o0), _ptr(o1), _ptr(o2),
+                                _stream(stream)))
+
+
+def carry128(hi, lo, N, out, n_out, out_bits=32, stream=None):
+    _check(lib().bw_carry128(_ptr(hi), _ptr(lo), N, _ptr(out), n_out, out_bits, _stream(stream)))
+
+
+def carry160(lo, mid, hi, N, out, n_out, stream=None):
+    _check(lib().bw_carry160(_ptr(lo), _ptr(mid), _ptr(hi), N, _ptr(out), n_out, _stream(stream)))

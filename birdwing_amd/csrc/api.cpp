@@ -1,0 +1,761 @@
+// api.cpp -- plans, orchestration and the C ABI of libbirdwing_hip.so.
+//
+// One plan = one (limb width, L_A, L_B, prime set).  A plan owns its residue
+// buffers, the carry-scan flags and per-plan constants; twiddle tables are
+// per (device, prime set) and shared.  bw_execute issues, on one stream and
+// with no host synchronization:
+//
+//   passes == 1:  FUSED(ingress A,B; fwd; pointwise*N^-1; inv)  -> Garner+carry
+//   passes >= 2:  FWD pass 1 (ingress, A and B) .. FWD pass P-1,
+//                 FUSED pass P (fwd A,B; pointwise; inv), INV passes P-1 .. 1,
+//                 Garner+carry
+//
+// The reference's orchestration (src/gpu_ntt.cu:442-973) instead launches
+// zero-pad, NTT, pointwise, INTT, CRT and a 3-pass carry separately with four
+// host synchronizations; the stage boundaries it reports are kept in the
+// per-launch timing (bw_plan_timing).
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstring>
+#include <cstdarg>
+#include <chrono>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/birdwing.h"
+#include "bw_internal.h"
+#include "carry_kernels.h"
+#include "host_math.h"
+#include "misc_kernels.h"
+#include "ntt_kernels.h"
+
+using namespace bw;
+
+// ---------------------------------------------------------------------------
+// errors
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) return fail(BW_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char* bw_last_error(void) { return g_err.c_str(); }
+extern "C" const char* bw_version(void) { return "birdwing-mi355x 0.1 (gfx950)"; }
+
+// ---------------------------------------------------------------------------
+// per (device, set) tables
+struct DevTables {
+    SetTables t;
+    GarnerDev garner;
+    F64Prime fp[3];
+};
+
+static std::mutex g_tab_mu;
+static std::map<std::pair<int, int>, std::unique_ptr<DevTables>> g_tabs;
+
+static int get_tables(int set_id, DevTables** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    auto key = std::make_pair(dev, set_id);
+    auto it = g_tabs.find(key);
+    if (it != g_tabs.end()) { *out = it->second.get(); return BW_OK; }
+    const PrimeSet* s = prime_set(set_id);
+    if (!s) return fail(BW_ERR_ARG, "unknown prime set %d", set_id);
+    HostTables h;
+    build_tables(*s, h);
+    std::unique_ptr<DevTables> d(new DevTables());
+    d->t.limb_bits = s->limb_bits;
+    d->t.k = s->k;
+    d->t.table_log = h.table_log;
+    void** ptrs[4] = {&d->t.consts, &d->t.tw_stage, &d->t.tw_lo, &d->t.tw_hi};
+    std::vector<unsigned char>* srcs[4] = {&h.consts, &h.tw_stage, &h.tw_lo, &h.tw_hi};
+    for (int i = 0; i < 4; i++) {
+        HIP_TRY(hipMalloc(ptrs[i], srcs[i]->size()));
+        HIP_TRY(hipMemcpy(*ptrs[i], srcs[i]->data(), srcs[i]->size(), hipMemcpyHostToDevice));
+    }
+    d->garner = garner_constants(*s);
+    for (int i = 0; i < 3; i++) f64_prime(i < s->k ? s->p[i] : 3, d->fp[i]);
+    *out = d.get();
+    g_tabs[key] = std::move(d);
+    return BW_OK;
+}
+
+extern "C" int bw_init(int device) {
+    HIP_TRY(hipSetDevice(device));
+    DevTables* t;
+    int rc = get_tables(SET_U32_REF, &t);
+    if (rc) return rc;
+    return get_tables(SET_U32_WIDE, &t);
+}
+
+// ---------------------------------------------------------------------------
+// size policy
+extern "C" uint64_t bw_padded_ntt_size(uint64_t L_A, uint64_t L_B) { return padded_ntt_size(L_A, L_B); }
+extern "C" int bw_default_prime_set(int limb_bits, uint64_t N) { return default_set(limb_bits, N); }
+
+extern "C" int bw_prime_set_info(int set, int* k, uint64_t* primes, uint64_t* roots) {
+    const PrimeSet* s = prime_set(set);
+    if (!s) return fail(BW_ERR_ARG, "unknown prime set %d", set);
+    if (k) *k = s->k;
+    for (int i = 0; i < s->k; i++) {
+        if (primes) primes[i] = s->p[i];
+        if (roots) roots[i] = s->root_max[i];
+    }
+    return BW_OK;
+}
+
+extern "C" int bw_max_supported_logN(int set) {
+    const PrimeSet* s = prime_set(set);
+    return s ? set_max_log(*s) : -1;
+}
+extern "C" uint64_t bw_max_limb_count_by_crt(int set, int in_bits) {
+    const PrimeSet* s = prime_set(set);
+    return s ? max_limb_count_by_crt(*s, in_bits) : 0;
+}
+extern "C" uint64_t bw_max_supported_limb_count(int set, int in_bits) {
+    const PrimeSet* s = prime_set(set);
+    return s ? max_supported_limb_count(*s, in_bits) : 0;
+}
+
+static void put_why(const std::string& w, char* why, size_t len) {
+    if (why && len) {
+        std::strncpy(why, w.c_str(), len - 1);
+        why[len - 1] = 0;
+    }
+}
+extern "C" int bw_ntt_size_supported(int set, uint64_t N, char* why, size_t len) {
+    const PrimeSet* s = prime_set(set);
+    if (!s) return 0;
+    std::string w;
+    bool ok = ntt_size_supported(*s, N, &w);
+    if (!ok) put_why(w, why, len);
+    return ok;
+}
+extern "C" int bw_crt_bound_satisfied(int set, int in_bits, uint64_t L_A, uint64_t L_B, char* why, size_t len) {
+    const PrimeSet* s = prime_set(set);
+    if (!s) return 0;
+    std::string w;
+    bool ok = crt_bound_ok(*s, in_bits, L_A, L_B, &w);
+    if (!ok) put_why(w, why, len);
+    return ok;
+}
+extern "C" int bw_multiply_size_supported(int set, int in_bits, uint64_t L_A, uint64_t L_B, char* why, size_t len) {
+    if (set < 0) set = default_set(in_bits, padded_ntt_size(L_A, L_B));
+    const PrimeSet* s = prime_set(set);
+    if (!s) return 0;
+    std::string w;
+    bool ok = multiply_size_supported(*s, in_bits, L_A, L_B, &w);
+    if (!ok) put_why(w, why, len);
+    return ok;
+}
+
+// ---------------------------------------------------------------------------
+// plans
+#define BW_MAX_LAUNCH 12
+
+struct bw_plan {
+    int limb_bits = 32;
+    int set_id = 0;
+    const PrimeSet* set = nullptr;
+    DevTables* tabs = nullptr;
+    uint64_t L_A = 0, L_B = 0, N = 0, N_exec = 0;
+    int logN = 0;
+    std::vector<int> passes;
+    size_t word = 4;
+    void* bufA = nullptr;
+    void* bufB = nullptr;
+    void* ninv = nullptr;
+    unsigned long long* flags = nullptr;
+    int* counter = nullptr;         // counter[0]: tile ids, counter[1]: look-back error word
+    int64_t ntiles = 0;
+    uint32_t epoch = 0;
+    uint64_t dev_bytes = 0;
+    // host execution
+    void* h_a = nullptr;
+    void* h_b = nullptr;
+    void* h_c = nullptr;
+    hipStream_t own = nullptr;
+    // timing
+    bool timing = false;
+    hipEvent_t ev[BW_MAX_LAUNCH + 1] = {};
+    int nev = 0;
+};
+
+static void default_passes(int logN, std::vector<int>& out) {
+    out.clear();
+    if (logN <= BW_MAX_PASS_LOG) { out.push_back(logN); return; }
+    if (logN <= 2 * BW_MAX_PASS_LOG) {
+        const int t2 = logN - 1 < BW_MAX_PASS_LOG ? logN - 1 : BW_MAX_PASS_LOG;
+        out.push_back(logN - t2);
+        out.push_back(t2);
+        return;
+    }
+    const int r = logN - BW_MAX_PASS_LOG;
+    const int t2 = r - 1 < BW_MAX_PASS_LOG ? r - 1 : BW_MAX_PASS_LOG;
+    out.push_back(r - t2);
+    out.push_back(t2);
+    out.push_back(BW_MAX_PASS_LOG);
+}
+
+static void plan_free(bw_plan* p) {
+    if (!p) return;
+    (void)hipFree(p->bufA);
+    (void)hipFree(p->bufB);
+    (void)hipFree(p->ninv);
+    (void)hipFree(p->flags);
+    (void)hipFree(p->counter);
+    (void)hipFree(p->h_a);
+    (void)hipFree(p->h_b);
+    (void)hipFree(p->h_c);
+    if (p->own) (void)hipStreamDestroy(p->own);
+    for (int i = 0; i <= BW_MAX_LAUNCH; i++)
+        if (p->ev[i]) (void)hipEventDestroy(p->ev[i]);
+    delete p;
+}
+
+static int plan_alloc(void** ptr, size_t bytes, bw_plan* p) {
+    hipError_t e = hipMalloc(ptr, bytes ? bytes : 16);
+    if (e != hipSuccess) return fail(BW_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    p->dev_bytes += bytes;
+    return BW_OK;
+}
+
+extern "C" int bw_plan_create_ex(bw_plan** out, int limb_bits, uint64_t L_A, uint64_t L_B, int set_id,
+                                 const int* pass_log, int npass) {
+    if (!out) return fail(BW_ERR_ARG, "plan pointer is NULL");
+    *out = nullptr;
+    if (limb_bits != 32 && limb_bits != 64) return fail(BW_ERR_ARG, "limb_bits must be 32 or 64");
+    if (L_A == 0 || L_B == 0) return fail(BW_ERR_UNSUPPORTED_SIZE, "operand limb count must be positive");
+    const uint64_t N = padded_ntt_size(L_A, L_B);
+    if (set_id < 0) set_id = default_set(limb_bits, N);
+    const PrimeSet* s = prime_set(set_id);
+    if (!s) return fail(BW_ERR_ARG, "unknown prime set %d", set_id);
+    if (s->limb_bits != limb_bits || s->k != 3)
+        return fail(BW_ERR_ARG, "prime set %d does not serve %d-bit limbs", set_id, limb_bits);
+    std::string why;
+    if (!multiply_size_supported(*s, limb_bits, L_A, L_B, &why))
+        return fail(BW_ERR_UNSUPPORTED_SIZE, "unsupported multiply: %s", why.c_str());
+    const uint64_t N_exec = N < 2 ? 2 : N;
+    const int logN = valuation2(N_exec);
+
+    bw_plan* p = new bw_plan();
+    p->limb_bits = limb_bits;
+    p->set_id = set_id;
+    p->set = s;
+    p->L_A = L_A;
+    p->L_B = L_B;
+    p->N = N;
+    p->N_exec = N_exec;
+    p->logN = logN;
+    p->word = limb_bits == 64 ? 8 : 4;
+    if (pass_log && npass > 0) {
+        int sum = 0;
+        for (int i = 0; i < npass; i++) {
+            if (pass_log[i] < 1 || pass_log[i] > BW_MAX_PASS_LOG) {
+                delete p;
+                return fail(BW_ERR_ARG, "pass_log[%d]=%d outside [1, %d]", i, pass_log[i], BW_MAX_PASS_LOG);
+            }
+            sum += pass_log[i];
+            p->passes.push_back(pass_log[i]);
+        }
+        if (sum != logN || npass > 4) {
+            delete p;
+            return fail(BW_ERR_ARG, "pass split does not add up to log2 N = %d (or more than 4 passes)", logN);
+        }
+    } else {
+        default_passes(logN, p->passes);
+    }
+    int rc = get_tables(set_id, &p->tabs);
+    if (rc) { delete p; return rc; }
+    const size_t plane_bytes = (size_t)N_exec * p->word * s->k;
+    if ((rc = plan_alloc(&p->bufA, plane_bytes, p)) || (rc = plan_alloc(&p->bufB, plane_bytes, p))) {
+        plan_free(p);
+        return rc;
+    }
+    std::vector<uint64_t> n64;
+    std::vector<double> nf;
+    ninv_pairs(*s, N_exec, n64, nf);
+    if ((rc = plan_alloc(&p->ninv, 16 * s->k, p))) { plan_free(p); return rc; }
+    hipError_t e = hipMemcpy(p->ninv, limb_bits == 64 ? (const void*)n64.data() : (const void*)nf.data(),
+                             16 * s->k, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { plan_free(p); return fail(BW_ERR_HIP, "hipMemcpy: %s", hipGetErrorString(e)); }
+    p->ntiles = bw_carry_tiles((int64_t)(L_A + L_B));
+    if ((rc = plan_alloc((void**)&p->flags, 8 * (size_t)p->ntiles, p)) ||
+        (rc = plan_alloc((void**)&p->counter, 2 * sizeof(int), p))) {
+        plan_free(p);
+        return rc;
+    }
+    if ((e = hipMemset(p->flags, 0, 8 * (size_t)p->ntiles)) != hipSuccess ||
+        (e = hipMemset(p->counter, 0, 2 * sizeof(int))) != hipSuccess) {
+        plan_free(p);
+        return fail(BW_ERR_HIP, "hipMemset: %s", hipGetErrorString(e));
+    }
+    *out = p;
+    return BW_OK;
+}
+
+extern "C" int bw_plan_create(bw_plan** out, int limb_bits, uint64_t L_A, uint64_t L_B, int set_id) {
+    return bw_plan_create_ex(out, limb_bits, L_A, L_B, set_id, nullptr, 0);
+}
+
+extern "C" void bw_plan_destroy(bw_plan* p) { plan_free(p); }
+
+static int count_launches(const bw_plan* p) {
+    const int P = (int)p->passes.size();
+    return P == 1 ? 2 : (P - 1) + 1 + (P - 1) + 1;
+}
+
+extern "C" int bw_plan_get_info(const bw_plan* p, bw_plan_info* info) {
+    if (!p || !info) return fail(BW_ERR_ARG, "NULL argument");
+    std::memset(info, 0, sizeof(*info));
+    info->limb_bits = p->limb_bits;
+    info->prime_set = p->set_id;
+    info->k = p->set->k;
+    for (int i = 0; i < p->set->k; i++) info->primes[i] = p->set->p[i];
+    info->L_A = p->L_A;
+    info->L_B = p->L_B;
+    info->N = p->N;
+    info->N_exec = p->N_exec;
+    info->npass = (int)p->passes.size();
+    for (int i = 0; i < info->npass && i < 4; i++) info->pass_log[i] = p->passes[i];
+    info->nlaunch = count_launches(p);
+    info->device_bytes = p->dev_bytes;
+    return BW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// execution
+struct PassPlan {
+    int logT;
+    int64_t S, O;
+    int logM;
+    int G;
+    int64_t tiles;
+};
+
+static PassPlan pass_geom(const bw_plan* p, int j) {
+    PassPlan g;
+    int prefix = 0;
+    for (int i = 0; i < j; i++) prefix += p->passes[i];
+    g.logT = p->passes[j];
+    g.logM = p->logN - prefix;
+    g.S = (int64_t)1 << (p->logN - prefix - g.logT);
+    g.O = (int64_t)1 << prefix;
+    const int gmax = bw_pass_max_G(g.logT);
+    if (g.S > 1) g.G = (int)(g.S < gmax ? g.S : gmax);
+    else g.G = (int)(g.O < gmax ? g.O : gmax);
+    g.tiles = g.O * g.S / g.G;
+    return g;
+}
+
+static PassParams base_params(const bw_plan* p, const PassPlan& g) {
+    PassParams prm;
+    std::memset(&prm, 0, sizeof(prm));
+    prm.N = (int64_t)p->N_exec;
+    prm.plane = (int64_t)p->N_exec;
+    prm.S = g.S;
+    prm.O = g.O;
+    prm.logM = g.logM;
+    prm.table_log = p->tabs->t.table_log;
+    prm.consts = p->tabs->t.consts;
+    prm.tw_stage = p->tabs->t.tw_stage;
+    prm.tw_lo = p->tabs->t.tw_lo;
+    prm.tw_hi = p->tabs->t.tw_hi;
+    prm.ninv = p->ninv;
+    prm.canon_out = 1;
+    return prm;
+}
+
+static hipError_t launch(const bw_plan* p, int mode, const PassPlan& g, const PassParams& prm, int nops,
+                         hipStream_t st) {
+    PassLaunch L;
+    L.mode = mode;
+    L.logT = g.logT;
+    L.G = g.G;
+    L.limb_bits = p->limb_bits;
+    L.nprimes = p->set->k;
+    L.nops = nops;
+    L.tiles = g.tiles;
+    return bw_launch_pass(L, prm, st);
+}
+
+struct Marker {
+    bw_plan* p;
+    hipStream_t st;
+    int n = 0;
+    hipError_t mark() {
+        if (!p->timing) return hipSuccess;
+        if (n > BW_MAX_LAUNCH) return hipSuccess;
+        hipError_t e = hipEventRecord(p->ev[n], st);
+        n++;
+        p->nev = n;
+        return e;
+    }
+};
+
+#define LAUNCH_TRY(expr)                                                                                 \
+    do {                                                                                                 \
+        hipError_t e_ = (expr);                                                                          \
+        if (e_ != hipSuccess) return fail(BW_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e_)); \
+    } while (0)
+
+// Transform + product + inverse, leaving canonical residues in bufA.
+static int run_transforms(bw_plan* p, const void* d_a, const void* d_b, hipStream_t st, Marker& mk) {
+    const int P = (int)p->passes.size();
+    if (P == 1) {
+        PassPlan g = pass_geom(p, 0);
+        PassParams prm = base_params(p, g);
+        prm.ingress = 1;
+        prm.in[0] = d_a;
+        prm.in[1] = d_b;
+        prm.in_len[0] = (int64_t)p->L_A;
+        prm.in_len[1] = (int64_t)p->L_B;
+        prm.out[0] = p->bufA;
+        LAUNCH_TRY(launch(p, BW_MODE_FUSED, g, prm, 1, st));
+        LAUNCH_TRY(mk.mark());
+        return BW_OK;
+    }
+    for (int j = 0; j < P - 1; j++) {
+        PassPlan g = pass_geom(p, j);
+        PassParams prm = base_params(p, g);
+        prm.nops = 2;
+        if (j == 0) {
+            prm.ingress = 1;
+            prm.in[0] = d_a;
+            prm.in[1] = d_b;
+            prm.in_len[0] = (int64_t)p->L_A;
+            prm.in_len[1] = (int64_t)p->L_B;
+        } else {
+            prm.in[0] = p->bufA;
+            prm.in[1] = p->bufB;
+        }
+        prm.out[0] = p->bufA;
+        prm.out[1] = p->bufB;
+        LAUNCH_TRY(launch(p, BW_MODE_FWD, g, prm, 2, st));
+        LAUNCH_TRY(mk.mark());
+    }
+    {
+        PassPlan g = pass_geom(p, P - 1);
+        PassParams prm = base_params(p, g);
+        prm.in[0] = p->bufA;
+        prm.in[1] = p->bufB;
+        prm.out[0] = p->bufA;
+        LAUNCH_TRY(launch(p, BW_MODE_FUSED, g, prm, 1, st));
+        LAUNCH_TRY(mk.mark());
+    }
+    for (int j = P - 2; j >= 0; j--) {
+        PassPlan g = pass_geom(p, j);
+        PassParams prm = base_params(p, g);
+        prm.in[0] = p->bufA;
+        prm.out[0] = p->bufA;
+        LAUNCH_TRY(launch(p, BW_MODE_INV, g, prm, 1, st));
+        LAUNCH_TRY(mk.mark());
+    }
+    return BW_OK;
+}
+
+static int check_plan(const bw_plan* p) {
+    if (!p) return fail(BW_ERR_ARG, "plan is NULL");
+    return BW_OK;
+}
+
+extern "C" int bw_execute(bw_plan* p, const void* d_a, const void* d_b, void* d_c, void* stream) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!d_a || !d_b || !d_c) return fail(BW_ERR_ARG, "NULL device buffer");
+    hipStream_t st = (hipStream_t)stream;
+    Marker mk{p, st};
+    p->nev = 0;
+    LAUNCH_TRY(mk.mark());
+    if ((rc = run_transforms(p, d_a, d_b, st, mk))) return rc;
+    CarryParams cp;
+    std::memset(&cp, 0, sizeof(cp));
+    cp.src = p->limb_bits == 64 ? BW_SRC_GARNER_U64 : BW_SRC_GARNER_U32;
+    cp.a0 = p->bufA;
+    cp.plane = (int64_t)p->N_exec;
+    cp.N = (int64_t)p->N_exec;
+    cp.n_out = (int64_t)(p->L_A + p->L_B);
+    cp.out = d_c;
+    cp.g = p->tabs->garner;
+    for (int i = 0; i < 3; i++) cp.fp[i] = p->tabs->fp[i];
+    cp.flags = p->flags;
+    cp.counter = p->counter;
+    cp.err = p->counter + 1;
+    cp.epoch = ++p->epoch;
+    cp.ntiles = p->ntiles;
+    LAUNCH_TRY(bw_launch_garner_carry(cp, st));
+    LAUNCH_TRY(mk.mark());
+    return BW_OK;
+}
+
+extern "C" int bw_execute_crt(bw_plan* p, const void* d_a, const void* d_b, uint64_t* d_c0, uint64_t* d_c1,
+                              uint32_t* d_c2, void* stream) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!d_a || !d_b || !d_c0 || !d_c1 || (p->limb_bits == 64 && !d_c2)) return fail(BW_ERR_ARG, "NULL device buffer");
+    hipStream_t st = (hipStream_t)stream;
+    Marker mk{p, st};
+    p->nev = 0;
+    LAUNCH_TRY(mk.mark());
+    if ((rc = run_transforms(p, d_a, d_b, st, mk))) return rc;
+    const int64_t N = (int64_t)p->N_exec;
+    const unsigned char* base = (const unsigned char*)p->bufA;
+    LAUNCH_TRY(bw_launch_garner_only(p->limb_bits, base, base + N * p->word, base + 2 * N * p->word, N,
+                                     p->tabs->garner, p->tabs->fp, d_c0, d_c1, d_c2, st));
+    LAUNCH_TRY(mk.mark());
+    return BW_OK;
+}
+
+static int ensure_host_buffers(bw_plan* p) {
+    if (p->h_a) return BW_OK;
+    const size_t wb = p->limb_bits / 8;
+    int rc;
+    if ((rc = plan_alloc(&p->h_a, p->L_A * wb, p)) || (rc = plan_alloc(&p->h_b, p->L_B * wb, p)) ||
+        (rc = plan_alloc(&p->h_c, (p->L_A + p->L_B) * wb, p)))
+        return rc;
+    HIP_TRY(hipStreamCreate(&p->own));
+    return BW_OK;
+}
+
+extern "C" int bw_execute_host(bw_plan* p, const void* a, const void* b, void* c) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if ((rc = ensure_host_buffers(p))) return rc;
+    const size_t wb = p->limb_bits / 8;
+    HIP_TRY(hipMemcpyAsync(p->h_a, a, p->L_A * wb, hipMemcpyHostToDevice, p->own));
+    HIP_TRY(hipMemcpyAsync(p->h_b, b, p->L_B * wb, hipMemcpyHostToDevice, p->own));
+    if ((rc = bw_execute(p, p->h_a, p->h_b, p->h_c, p->own))) return rc;
+    HIP_TRY(hipMemcpyAsync(c, p->h_c, (p->L_A + p->L_B) * wb, hipMemcpyDeviceToHost, p->own));
+    HIP_TRY(hipStreamSynchronize(p->own));
+    return BW_OK;
+}
+
+extern "C" int bw_plan_status(bw_plan* p) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    int err = 0;
+    HIP_TRY(hipMemcpy(&err, p->counter + 1, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) return fail(BW_ERR_STATE, "carry look-back wait exceeded its bound");
+    return BW_OK;
+}
+
+extern "C" int bw_plan_set_timing(bw_plan* p, int enabled) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (enabled && !p->ev[0])
+        for (int i = 0; i <= BW_MAX_LAUNCH; i++) HIP_TRY(hipEventCreate(&p->ev[i]));
+    p->timing = enabled != 0;
+    return BW_OK;
+}
+
+extern "C" int bw_plan_timing(const bw_plan* p, float* ms, int max, int* n) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    const int cnt = p->nev > 0 ? p->nev - 1 : 0;
+    if (n) *n = cnt;
+    if (cnt == 0) return BW_OK;
+    HIP_TRY(hipEventSynchronize(p->ev[cnt]));
+    for (int i = 0; i < cnt && i < max; i++) HIP_TRY(hipEventElapsedTime(&ms[i], p->ev[i], p->ev[i + 1]));
+    return BW_OK;
+}
+
+extern "C" int bw_multiply_host(int limb_bits, const void* a, uint64_t L_A, const void* b, uint64_t L_B, void* c,
+                                double* ms) {
+    auto t0 = std::chrono::steady_clock::now();
+    bw_plan* p = nullptr;
+    int rc = bw_plan_create(&p, limb_bits, L_A, L_B, -1);
+    if (rc) return rc;
+    rc = bw_execute_host(p, a, b, c);
+    bw_plan_destroy(p);
+    if (ms) *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// stage-level entry points
+extern "C" int bw_forward(bw_plan* p, const void* d_raw, uint64_t L, void* d_spec, void* stream) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!d_raw || !d_spec) return fail(BW_ERR_ARG, "NULL device buffer");
+    hipStream_t st = (hipStream_t)stream;
+    const int P = (int)p->passes.size();
+    for (int j = 0; j < P; j++) {
+        PassPlan g = pass_geom(p, j);
+        if (j == P - 1 && g.S == 1 && P == 1) g.G = 1;
+        PassParams prm = base_params(p, g);
+        prm.nops = 1;
+        if (j == 0) {
+            prm.ingress = 1;
+            prm.in[0] = d_raw;
+            prm.in_len[0] = (int64_t)L;
+        } else {
+            prm.in[0] = d_spec;
+        }
+        prm.out[0] = d_spec;
+        LAUNCH_TRY(launch(p, BW_MODE_FWD, g, prm, 1, st));
+    }
+    return BW_OK;
+}
+
+extern "C" int bw_inverse(bw_plan* p, void* d_spec, void* stream) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!d_spec) return fail(BW_ERR_ARG, "NULL device buffer");
+    hipStream_t st = (hipStream_t)stream;
+    const int P = (int)p->passes.size();
+    for (int j = P - 1; j >= 0; j--) {
+        PassPlan g = pass_geom(p, j);
+        PassParams prm = base_params(p, g);
+        prm.in[0] = d_spec;
+        prm.out[0] = d_spec;
+        LAUNCH_TRY(launch(p, BW_MODE_INV, g, prm, 1, st));
+    }
+    LAUNCH_TRY(bw_launch_scale(p->limb_bits, d_spec, (int64_t)p->N_exec, (int64_t)p->N_exec, p->set->k,
+                               p->tabs->t.consts, p->ninv, st));
+    return BW_OK;
+}
+
+extern "C" int bw_spectrum_order(const bw_plan* p, uint64_t* order) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    const int P = (int)p->passes.size();
+    std::vector<int64_t> S(P);
+    int prefix = 0;
+    for (int j = 0; j < P; j++) {
+        prefix += p->passes[j];
+        S[j] = (int64_t)1 << (p->logN - prefix);
+    }
+    for (uint64_t q = 0; q < p->N_exec; q++) {
+        uint64_t f = 0, mult = 1;
+        for (int j = 0; j < P; j++) {
+            const int64_t qj = (int64_t)(q / S[j]) & (((int64_t)1 << p->passes[j]) - 1);
+            f += (uint64_t)sched_freq_of_pos(p->passes[j], qj) * mult;
+            mult <<= p->passes[j];
+        }
+        order[q] = f;
+    }
+    return BW_OK;
+}
+
+extern "C" int bw_zero_pad(const uint32_t* src, void* dst, int word_bytes, uint64_t L, uint64_t N, void* stream) {
+    if (!src || !dst || (word_bytes != 4 && word_bytes != 8)) return fail(BW_ERR_ARG, "bad zero_pad arguments");
+    LAUNCH_TRY(bw_launch_zero_pad(src, dst, word_bytes, (int64_t)L, (int64_t)N, (hipStream_t)stream));
+    return BW_OK;
+}
+
+extern "C" int bw_zero_pad_u64(const uint64_t* src, uint64_t* dst, uint64_t L, uint64_t N, uint64_t p, void* stream) {
+    if (!src || !dst || p < 3 || p >= (1ULL << 62)) return fail(BW_ERR_ARG, "bad zero_pad_u64 arguments");
+    U64Prime P;
+    std::memset(&P, 0, sizeof(P));
+    P.p = p;
+    P.floor_r = ~0ULL / p;
+    LAUNCH_TRY(bw_launch_zero_pad_u64(src, dst, (int64_t)L, (int64_t)N, P, (hipStream_t)stream));
+    return BW_OK;
+}
+
+extern "C" int bw_pointwise(int word_bytes, const void* a, const void* b, void* c, uint64_t N, uint64_t p,
+                            void* stream) {
+    if (!a || !b || !c || (word_bytes != 4 && word_bytes != 8) || p < 3) return fail(BW_ERR_ARG, "bad pointwise arguments");
+    if (word_bytes == 4 && p >= (1ULL << 31)) return fail(BW_ERR_ARG, "32-bit words need p < 2^31");
+    if (word_bytes == 4 && (((p - 1) >> valuation2(p - 1)) >= 128))
+        return fail(BW_ERR_ARG, "32-bit words need p = c*2^m + 1 with c < 128");
+    LAUNCH_TRY(bw_launch_pointwise(word_bytes, a, b, c, (int64_t)N, p, (hipStream_t)stream));
+    return BW_OK;
+}
+
+extern "C" int bw_crt_combine(int set_id, const void* r0, const void* r1, const void* r2, uint64_t N, uint64_t* o0,
+                              uint64_t* o1, uint32_t* o2, void* stream) {
+    const PrimeSet* s = prime_set(set_id);
+    if (!s || s->k != 3) return fail(BW_ERR_ARG, "prime set %d has no 3-prime Garner", set_id);
+    if (!r0 || !r1 || !r2 || !o0 || !o1 || (s->limb_bits == 64 && !o2)) return fail(BW_ERR_ARG, "NULL buffer");
+    DevTables* t;
+    int rc = get_tables(set_id, &t);
+    if (rc) return rc;
+    LAUNCH_TRY(bw_launch_garner_only(s->limb_bits, r0, r1, r2, (int64_t)N, t->garner, t->fp, o0, o1, o2,
+                                     (hipStream_t)stream));
+    return BW_OK;
+}
+
+// Scratch of the standalone carry entry points: flags + counter per device,
+// grown on demand.  Calls that overlap on one device serialize on the mutex
+// only for the launch; the stream order keeps their flags apart by epoch.
+struct CarryScratch {
+    unsigned long long* flags = nullptr;
+    int* counter = nullptr;
+    int64_t cap = 0;
+    uint32_t epoch = 0;
+};
+static std::mutex g_carry_mu;
+static std::map<int, CarryScratch> g_carry;
+
+static int standalone_carry(CarryParams& cp, hipStream_t st) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_carry_mu);
+    CarryScratch& cs = g_carry[dev];
+    cp.ntiles = bw_carry_tiles(cp.n_out);
+    if (cp.ntiles == 0) return BW_OK;
+    if (cs.cap < cp.ntiles) {
+        HIP_TRY(hipStreamSynchronize(st));
+        HIP_TRY(hipDeviceSynchronize());
+        (void)hipFree(cs.flags);
+        if (!cs.counter) {
+            HIP_TRY(hipMalloc(&cs.counter, 2 * sizeof(int)));
+            HIP_TRY(hipMemset(cs.counter, 0, 2 * sizeof(int)));
+        }
+        HIP_TRY(hipMalloc(&cs.flags, 8 * (size_t)cp.ntiles));
+        HIP_TRY(hipMemset(cs.flags, 0, 8 * (size_t)cp.ntiles));
+        cs.cap = cp.ntiles;
+    }
+    cp.flags = cs.flags;
+    cp.counter = cs.counter;
+    cp.err = cs.counter + 1;
+    cp.epoch = ++cs.epoch;
+    LAUNCH_TRY(bw_launch_garner_carry(cp, st));
+    return BW_OK;
+}
+
+extern "C" int bw_carry128(const uint64_t* hi, const uint64_t* lo, uint64_t N, void* out, uint64_t n_out,
+                           int out_bits, void* stream) {
+    if (!hi || !lo || !out || (out_bits != 32 && out_bits != 64)) return fail(BW_ERR_ARG, "bad carry128 arguments");
+    CarryParams cp;
+    std::memset(&cp, 0, sizeof(cp));
+    cp.src = out_bits == 64 ? BW_SRC_HILO128_L64 : BW_SRC_HILO128;
+    cp.a0 = hi;
+    cp.a1 = lo;
+    cp.N = (int64_t)N;
+    cp.n_out = (int64_t)n_out;
+    cp.out = out;
+    return standalone_carry(cp, (hipStream_t)stream);
+}
+
+extern "C" int bw_carry160(const uint64_t* lo, const uint64_t* mid, const uint32_t* hi, uint64_t N, void* out,
+                           uint64_t n_out, void* stream) {
+    if (!lo || !mid || !hi || !out) return fail(BW_ERR_ARG, "bad carry160 arguments");
+    CarryParams cp;
+    std::memset(&cp, 0, sizeof(cp));
+    cp.src = BW_SRC_U160;
+    cp.a0 = lo;
+    cp.a1 = mid;
+    cp.a2 = hi;
+    cp.N = (int64_t)N;
+    cp.n_out = (int64_t)n_out;
+    cp.out = out;
+    return standalone_carry(cp, (hipStream_t)stream);
+}

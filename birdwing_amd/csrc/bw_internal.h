@@ -1,0 +1,85 @@
+// bw_internal.h -- structures shared by the host orchestration and kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include "arith.h"
+
+#define BW_MAX_PRIMES 3
+#define BW_STAGE_LOG 13          // stage twiddle table: omega_{2^13}^e, e < 2^13
+#define BW_TW_LO_LOG 12          // pass twiddles: 2-level table split at 2^12
+#define BW_MAX_PASS_LOG 13       // largest sub-transform held in one workgroup
+
+// Per-prime constants of the FP64 (u32 digit) path.
+struct F64PrimeDev {
+    F64Prime P;
+    double w16[8][2];    // omega_16^e, e < 8: (w, w/p)
+    double iw16[8][2];   // omega_16^-e
+};
+
+// Per-prime constants of the u64 digit path (Shoup pairs (w, floor(w 2^64/p))).
+struct U64PrimeDev {
+    U64Prime P;
+    uint64_t w16[8][2];
+    uint64_t iw16[8][2];
+};
+
+// Device tables for one prime set, shared by every plan that uses the set.
+struct SetTables {
+    int limb_bits;         // 32 (FP64 arithmetic) or 64 (integer arithmetic)
+    int k;                 // number of primes
+    int table_log;         // omega_{2^table_log} is the table root
+    void* consts;          // F64PrimeDev[k] or U64PrimeDev[k]
+    void* tw_stage;        // [k][2^BW_STAGE_LOG] pairs: omega_{2^13}^e
+    void* tw_lo;           // [k][2^BW_TW_LO_LOG] pairs: omega_{2^L}^e
+    void* tw_hi;           // [k][2^(L-12)] pairs: omega_{2^L}^(e*2^12)
+};
+
+// Garner constants for the fused CRT+carry kernel.
+struct GarnerDev {
+    int k;
+    uint64_t p[3];
+    // u32 path (FP64 arithmetic): inverses as (w, w/p) pairs
+    double inv1[2], inv2[2], p0mod2[2];
+    // u64 path (Shoup pairs)
+    uint64_t inv1_s[2], inv2_s[2], p0mod2_s[2];
+    uint64_t p0;           // p0
+    uint64_t p01_lo, p01_hi;   // p0*p1 as 128-bit
+};
+
+// One pass of the multi-pass transform.  Element (o, t, s) of the N-array lives
+// at (o*T + t)*S + s; the pass transforms over t for every (o, s).
+struct PassGeom {
+    int logT;
+    int64_t S;        // stride
+    int64_t O;        // outer count
+    int logM;         // log2(T*S): sub-problem whose twiddles follow this pass
+    int G;            // lines per workgroup
+};
+
+enum { BW_MODE_FWD = 0, BW_MODE_INV = 1, BW_MODE_FUSED = 2 };
+
+struct PassParams {
+    // operands: FWD handles nops operands (A then B); FUSED reads both, writes out0;
+    // INV reads/writes operand 0.
+    const void* in[2];
+    void* out[2];
+    int64_t in_len[2];     // raw limb counts when ingress != 0
+    int ingress;           // 1: inputs are raw limbs (zero-padded to N, pass 1 only)
+    int canon_out;         // 1: store canonical residues in [0, p)
+    int nops;              // FWD: operands in this launch
+    int64_t N;             // transform length
+    int64_t plane;         // elements per prime plane (N, or batch stride)
+    int64_t S, O;
+    int logM;
+    int table_log;
+    int prime0;            // first prime index handled (grid.y adds to it)
+    const void* consts;    // F64PrimeDev* / U64PrimeDev*
+    const void* tw_stage;
+    const void* tw_lo;
+    const void* tw_hi;
+    const void* ninv;      // per-prime (N^-1, aux) pairs for FUSED mode
+};
+
+// Status codes (BW_OK, BW_ERR_*) come from the public header.
+#include "../../include/birdwing.h"

@@ -1,0 +1,50 @@
+// carry_kernels.h -- Garner CRT fused with a single-pass carry scan.
+//
+// A product coefficient c_n (< 2^(NW*LB)) is split into limb-width words
+// w_0..w_{NW-1}.  The output value is sum_n s_n 2^(LB n) with
+//     s_n = sum_j w_j(c_{n-j}),   s_n < NW * 2^LB,
+// so s_n = lo_n + hi_n 2^LB with hi_n < NW.  Folding hi_{n-1} into lo_n gives
+// t_n = lo_n + hi_{n-1} <= 2^LB + NW - 2, whose carry into the next limb is
+// binary: generate g_n = (t_n >= 2^LB), propagate p_n = (t_n == 2^LB - 1).
+// The carries then follow c_{n+1} = g_n | (p_n & c_n), which a wavefront
+// resolves for 64 limbs at once with two ballots and one 64-bit add
+// (carries = (X + Y + cin) ^ X ^ Y with X = G|P, Y = G), and workgroups chain
+// with a decoupled look-back over (generate, propagate) tile summaries
+// (Merrill & Garland 2016, "Single-pass Parallel Prefix Scan with Decoupled
+// Look-back"), tile ids taken from an atomic counter so every predecessor is
+// already running.
+#pragma once
+#include "bw_internal.h"
+
+enum {
+    BW_SRC_GARNER_U32 = 0,   // 3 canonical u32 residue planes -> 32-bit limbs
+    BW_SRC_GARNER_U64 = 1,   // 3 canonical u64 residue planes -> 64-bit limbs
+    BW_SRC_HILO128 = 2,      // (hi, lo) u64 arrays -> 32-bit limbs
+    BW_SRC_U160 = 3,         // (lo, mid, hi32) arrays -> 64-bit limbs
+    BW_SRC_HILO128_L64 = 4,  // (hi, lo) u64 arrays -> 64-bit limbs
+};
+
+struct CarryParams {
+    int src;
+    const void* a0;       // residues base / hi / lo arrays (see src)
+    const void* a1;
+    const void* a2;
+    int64_t plane;        // residue plane stride (elements)
+    int64_t N;            // coefficients (c_n = 0 for n >= N)
+    int64_t n_out;        // output limbs
+    void* out;
+    GarnerDev g;
+    F64Prime fp[3];       // FP64 constants of the three primes (u32 path)
+    unsigned long long* flags;   // one per tile
+    int* counter;         // tile-id allocator, returns to 0 after the launch
+    uint32_t epoch;       // tags this launch's flags
+    int64_t ntiles;
+    int* err;             // set to 1 if a look-back wait exceeded its bound
+};
+
+hipError_t bw_launch_garner_carry(const CarryParams& p, hipStream_t stream);
+int64_t bw_carry_tiles(int64_t n_out);
+// Standalone Garner (coefficients out): u32 path -> (hi, lo); u64 path -> U160.
+hipError_t bw_launch_garner_only(int limb_bits, const void* r0, const void* r1, const void* r2, int64_t N,
+                                 const GarnerDev& g, const F64Prime* fp,
+                                 uint64_t* o0, uint64_t* o1, uint32_t* o2, hipStream_t stream);

@@ -1,0 +1,324 @@
+// carry_kernels.hip -- Garner CRT + single-pass carry scan (see carry_kernels.h).
+#include "carry_kernels.h"
+
+#define BW_CARRY_ROWS 8                                   // rows of 64 limbs per wavefront
+#define BW_CARRY_TILE (4 * 64 * BW_CARRY_ROWS)            // limbs per 256-thread workgroup
+
+// ---------------------------------------------------------------------------
+// Garner's mixed-radix reconstruction x = k0 + p0 k1 + p0 p1 k2 with
+// k1 = (r1 - k0) p0^-1 mod p1 and k2 = (r2 - (k0 + p0 k1)) (p0 p1)^-1 mod p2:
+// the recurrence of the reference kernel (src/crt_gpu.cu:157-203), with the
+// x mod p_j bookkeeping done by the same M_mod_table products.
+
+// u32 path: residues < 2^31, FP64 modular products; x < 2^96 -> 3 words.
+__device__ __forceinline__ void garner_u32(uint32_t r0, uint32_t r1, uint32_t r2, const GarnerDev& g,
+                                           const F64Prime* fp, uint64_t& lo, uint32_t& hi32) {
+    const double d1 = (double)r1 - (double)r0;
+    const uint32_t k1 = f64_canon(f64_mulmod(d1, g.inv1[0], g.inv1[1], fp[1]), fp[1]);
+    const double xm = (double)r0 + f64_mulmod((double)k1, g.p0mod2[0], g.p0mod2[1], fp[2]);
+    const uint32_t k2 = f64_canon(f64_mulmod((double)r2 - xm, g.inv2[0], g.inv2[1], fp[2]), fp[2]);
+    const uint64_t v = (uint64_t)r0 + g.p0 * (uint64_t)k1;     // < 2^63
+    const uint64_t plo = g.p01_lo * (uint64_t)k2;
+    const uint64_t phi = __umul64hi(g.p01_lo, (uint64_t)k2);
+    lo = v + plo;
+    hi32 = (uint32_t)(phi + (lo < v ? 1ULL : 0ULL));
+}
+
+// u64 path: residues < p_j < 2^62, Shoup products; x < 2^174 in general but
+// < 2^160 for every supported size -> 3 words (lo, mid, hi).
+__device__ __forceinline__ uint64_t cond_sub(uint64_t x, uint64_t p) { return x >= p ? x - p : x; }
+
+__device__ __forceinline__ void garner_u64(uint64_t r0, uint64_t r1, uint64_t r2, const GarnerDev& g,
+                                           uint64_t& w0, uint64_t& w1, uint64_t& w2) {
+    const uint64_t p0 = g.p[0], p1 = g.p[1], p2 = g.p[2];
+    const uint64_t k1 = u64_mulmod_shoup(u64_sub(r1, cond_sub(r0, p1), p1), g.inv1_s[0], g.inv1_s[1], p1);
+    const uint64_t xm = u64_add(cond_sub(r0, p2),
+                                u64_mulmod_shoup(cond_sub(k1, p2), g.p0mod2_s[0], g.p0mod2_s[1], p2), p2);
+    const uint64_t k2 = u64_mulmod_shoup(u64_sub(r2, xm, p2), g.inv2_s[0], g.inv2_s[1], p2);
+    typedef unsigned __int128 u128;
+    u128 a = (u128)p0 * k1 + r0;                     // < 2^117
+    u128 b = (u128)g.p01_lo * k2;                    // low product
+    u128 c = (u128)g.p01_hi * k2;                    // shifted by 64
+    u128 s0 = (u128)(uint64_t)a + (uint64_t)b;
+    w0 = (uint64_t)s0;
+    u128 s1 = (s0 >> 64) + (a >> 64) + (b >> 64) + (uint64_t)c;
+    w1 = (uint64_t)s1;
+    w2 = (uint64_t)((s1 >> 64) + (c >> 64));
+}
+
+// ---------------------------------------------------------------------------
+// Coefficient sources: words w_j(c_n) in limb width, zero outside [0, N).
+struct SrcGarnerU32 {
+    using Word = uint32_t;
+    static constexpr int NW = 3;
+    __device__ static void load(const CarryParams& p, int64_t n, Word* w) {
+        if (n < 0 || n >= p.N) { w[0] = w[1] = w[2] = 0; return; }
+        const uint32_t* r = reinterpret_cast<const uint32_t*>(p.a0);
+        uint64_t lo; uint32_t hi;
+        garner_u32(r[n], r[p.plane + n], r[2 * p.plane + n], p.g, p.fp, lo, hi);
+        w[0] = (uint32_t)lo; w[1] = (uint32_t)(lo >> 32); w[2] = hi;
+    }
+};
+struct SrcGarnerU64 {
+    using Word = uint64_t;
+    static constexpr int NW = 3;
+    __device__ static void load(const CarryParams& p, int64_t n, Word* w) {
+        if (n < 0 || n >= p.N) { w[0] = w[1] = w[2] = 0; return; }
+        const uint64_t* r = reinterpret_cast<const uint64_t*>(p.a0);
+        garner_u64(r[n], r[p.plane + n], r[2 * p.plane + n], p.g, w[0], w[1], w[2]);
+    }
+};
+struct SrcHiLo128 {     // a0 = hi, a1 = lo
+    using Word = uint32_t;
+    static constexpr int NW = 4;
+    __device__ static void load(const CarryParams& p, int64_t n, Word* w) {
+        if (n < 0 || n >= p.N) { w[0] = w[1] = w[2] = w[3] = 0; return; }
+        const uint64_t hi = reinterpret_cast<const uint64_t*>(p.a0)[n];
+        const uint64_t lo = reinterpret_cast<const uint64_t*>(p.a1)[n];
+        w[0] = (uint32_t)lo; w[1] = (uint32_t)(lo >> 32); w[2] = (uint32_t)hi; w[3] = (uint32_t)(hi >> 32);
+    }
+};
+struct SrcHiLo128L64 {  // a0 = hi, a1 = lo, 64-bit limbs
+    using Word = uint64_t;
+    static constexpr int NW = 2;
+    __device__ static void load(const CarryParams& p, int64_t n, Word* w) {
+        if (n < 0 || n >= p.N) { w[0] = w[1] = 0; return; }
+        w[1] = reinterpret_cast<const uint64_t*>(p.a0)[n];
+        w[0] = reinterpret_cast<const uint64_t*>(p.a1)[n];
+    }
+};
+struct SrcU160 {        // a0 = lo, a1 = mid, a2 = hi32
+    using Word = uint64_t;
+    static constexpr int NW = 3;
+    __device__ static void load(const CarryParams& p, int64_t n, Word* w) {
+        if (n < 0 || n >= p.N) { w[0] = w[1] = w[2] = 0; return; }
+        w[0] = reinterpret_cast<const uint64_t*>(p.a0)[n];
+        w[1] = reinterpret_cast<const uint64_t*>(p.a1)[n];
+        w[2] = reinterpret_cast<const uint32_t*>(p.a2)[n];
+    }
+};
+
+// Lane l receives x from lane l-j of this row; lanes < j take prev from lane l+64-j.
+template <class Wd>
+__device__ __forceinline__ Wd shift_up(Wd cur, Wd prev, int j, int lane) {
+    const Wd a = __shfl(cur, (lane - j) & 63);
+    const Wd b = __shfl(prev, (lane + 64 - j) & 63);
+    return lane >= j ? a : b;
+}
+
+// Flag word: [63:32] epoch, [3] propagate, [2] generate/carry, [1:0] status.
+#define FLAG_AGG 1ULL
+#define FLAG_INC 2ULL
+
+template <class SRC>
+__global__ __launch_bounds__(256) void garner_carry_kernel(CarryParams prm) {
+    using Wd = typename SRC::Word;
+    constexpr int NW = SRC::NW;
+    constexpr int K = BW_CARRY_ROWS;
+    constexpr Wd ONES = (Wd)~(Wd)0;
+    __shared__ int64_t s_tile;
+    __shared__ uint32_t s_wagg[4];
+    __shared__ uint32_t s_wcin[4];
+
+    if (threadIdx.x == 0) {
+        const int t = atomicAdd(prm.counter, 1);
+        if ((int64_t)t == prm.ntiles - 1) atomicExch(prm.counter, 0);   // every id is taken: reset for the next launch
+        s_tile = t;
+    }
+    __syncthreads();
+    const int64_t tile = s_tile;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int64_t wbase = tile * BW_CARRY_TILE + (int64_t)wave * 64 * K;
+
+    Wd tval[K];
+    bool tg[K], tp[K];
+    Wd pw[NW];
+    Wd phi = 0;
+    // Row -1 supplies the lower neighbours of row 0 (its own lanes < NW-1 are never used).
+    {
+        Wd cw[NW];
+        SRC::load(prm, wbase - 64 + lane, cw);
+        Wd zero[NW];
+#pragma unroll
+        for (int j = 0; j < NW; j++) zero[j] = 0;
+        Wd lo = cw[0];
+        Wd hi = 0;
+#pragma unroll
+        for (int j = 1; j < NW; j++) {
+            const Wd v = shift_up(cw[j], zero[j], j, lane);
+            const Wd s = lo + v;
+            hi += (s < lo) ? 1 : 0;
+            lo = s;
+        }
+#pragma unroll
+        for (int j = 0; j < NW; j++) pw[j] = cw[j];
+        phi = hi;
+    }
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+        Wd cw[NW];
+        SRC::load(prm, wbase + i * 64 + lane, cw);
+        Wd lo = cw[0];
+        Wd hi = 0;
+#pragma unroll
+        for (int j = 1; j < NW; j++) {
+            const Wd v = shift_up(cw[j], pw[j], j, lane);
+            const Wd s = lo + v;
+            hi += (s < lo) ? 1 : 0;
+            lo = s;
+        }
+        const Wd h1 = shift_up(hi, phi, 1, lane);
+        const Wd t = lo + h1;
+        tg[i] = t < lo;                        // carry out of the limb
+        tp[i] = !tg[i] && (t == ONES);
+        tval[i] = t;
+#pragma unroll
+        for (int j = 0; j < NW; j++) pw[j] = cw[j];
+        phi = hi;
+    }
+
+    // Wave summary over its K rows (cin = 0).
+    uint32_t wgen = 0, wprop = 1;
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+        const uint64_t G = __ballot(tg[i]);
+        const uint64_t P = __ballot(tp[i]);
+        const uint64_t X = G | P;
+        const uint64_t sum = X + G;
+        const uint32_t rgen = sum < X ? 1u : 0u;
+        const uint32_t rprop = (P == ~0ULL) ? 1u : 0u;
+        wgen = rgen | (rprop & wgen);
+        wprop &= rprop;
+    }
+    if (lane == 0) s_wagg[wave] = wgen | (wprop << 1);
+    __syncthreads();
+
+    if (threadIdx.x == 0) {
+        uint32_t gen = 0, prop = 1;
+        for (int w = 0; w < 4; w++) {
+            const uint32_t a = s_wagg[w];
+            gen = (a & 1) | ((a >> 1) & gen);
+            prop &= (a >> 1);
+        }
+        const unsigned long long ep = (unsigned long long)prm.epoch << 32;
+        uint32_t cin = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&prm.flags[tile], ep | FLAG_INC | ((unsigned long long)gen << 2),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&prm.flags[tile], ep | FLAG_AGG | ((unsigned long long)gen << 2) |
+                                   ((unsigned long long)prop << 3),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t agen = 0, aprop = 1;
+            int64_t pred = tile - 1;
+            uint32_t spins = 0;
+            for (;;) {
+                const unsigned long long f =
+                    __hip_atomic_load(&prm.flags[pred], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((uint32_t)(f >> 32) != prm.epoch || (f & 3ULL) == 0) {
+                    // Predecessors publish their summary without waiting, so this
+                    // wait is short; the bound only turns a logic error into a
+                    // reported failure instead of a hang.
+                    if (++spins > (1u << 24)) {
+                        __hip_atomic_store(prm.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        cin = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                const uint32_t fg = (uint32_t)(f >> 2) & 1u;
+                if ((f & 3ULL) == FLAG_INC) {
+                    cin = agen | (aprop & fg);
+                    break;
+                }
+                const uint32_t fp = (uint32_t)(f >> 3) & 1u;
+                agen = agen | (aprop & fg);
+                aprop = aprop & fp;
+                if (!aprop) { cin = agen; break; }   // the chain cannot reach further back
+                pred--;
+            }
+            const uint32_t cout = gen | (prop & cin);
+            __hip_atomic_store(&prm.flags[tile], ep | FLAG_INC | ((unsigned long long)cout << 2),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        uint32_t c = cin;
+        for (int w = 0; w < 4; w++) {
+            s_wcin[w] = c;
+            const uint32_t a = s_wagg[w];
+            c = (a & 1) | ((a >> 1) & c);
+        }
+    }
+    __syncthreads();
+
+    uint64_t cin = s_wcin[wave];
+    Wd* out = reinterpret_cast<Wd*>(prm.out);
+#pragma unroll
+    for (int i = 0; i < K; i++) {
+        const uint64_t G = __ballot(tg[i]);
+        const uint64_t P = __ballot(tp[i]);
+        const uint64_t X = G | P;
+        const uint64_t s1 = X + G;
+        const uint64_t s2 = s1 + cin;
+        const uint64_t cout = (s1 < X || s2 < s1) ? 1ULL : 0ULL;
+        const uint64_t carries = s2 ^ P;           // carry into each lane
+        const Wd limb = tval[i] + (Wd)((carries >> lane) & 1ULL);
+        const int64_t n = wbase + i * 64 + lane;
+        if (n < prm.n_out) out[n] = limb;
+        cin = cout;
+    }
+}
+
+int64_t bw_carry_tiles(int64_t n_out) { return (n_out + BW_CARRY_TILE - 1) / BW_CARRY_TILE; }
+
+hipError_t bw_launch_garner_carry(const CarryParams& p, hipStream_t stream) {
+    dim3 grid((unsigned)p.ntiles), block(256);
+    switch (p.src) {
+    case BW_SRC_GARNER_U32: hipLaunchKernelGGL(garner_carry_kernel<SrcGarnerU32>, grid, block, 0, stream, p); break;
+    case BW_SRC_GARNER_U64: hipLaunchKernelGGL(garner_carry_kernel<SrcGarnerU64>, grid, block, 0, stream, p); break;
+    case BW_SRC_HILO128: hipLaunchKernelGGL(garner_carry_kernel<SrcHiLo128>, grid, block, 0, stream, p); break;
+    case BW_SRC_HILO128_L64: hipLaunchKernelGGL(garner_carry_kernel<SrcHiLo128L64>, grid, block, 0, stream, p); break;
+    case BW_SRC_U160: hipLaunchKernelGGL(garner_carry_kernel<SrcU160>, grid, block, 0, stream, p); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Standalone Garner: coefficients out (the reference's crt_combine_gpu layouts).
+__global__ void garner_u32_kernel(const uint32_t* r0, const uint32_t* r1, const uint32_t* r2, int64_t N,
+                                  GarnerDev g, F64Prime f0, F64Prime f1, F64Prime f2, uint64_t* hi, uint64_t* lo) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    const F64Prime fp[3] = {f0, f1, f2};
+    uint64_t l; uint32_t h;
+    garner_u32(r0[n], r1[n], r2[n], g, fp, l, h);
+    lo[n] = l;
+    hi[n] = h;
+}
+
+__global__ void garner_u64_kernel(const uint64_t* r0, const uint64_t* r1, const uint64_t* r2, int64_t N,
+                                  GarnerDev g, uint64_t* lo, uint64_t* mid, uint32_t* hi) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    uint64_t w0, w1, w2;
+    garner_u64(r0[n], r1[n], r2[n], g, w0, w1, w2);
+    lo[n] = w0;
+    mid[n] = w1;
+    hi[n] = (uint32_t)w2;
+}
+
+hipError_t bw_launch_garner_only(int limb_bits, const void* r0, const void* r1, const void* r2, int64_t N,
+                                 const GarnerDev& g, const F64Prime* fp,
+                                 uint64_t* o0, uint64_t* o1, uint32_t* o2, hipStream_t stream) {
+    if (N <= 0) return hipSuccess;
+    dim3 grid((unsigned)((N + 255) / 256)), block(256);
+    if (limb_bits == 64)
+        hipLaunchKernelGGL(garner_u64_kernel, grid, block, 0, stream, (const uint64_t*)r0, (const uint64_t*)r1,
+                           (const uint64_t*)r2, N, g, o0, o1, o2);
+    else
+        hipLaunchKernelGGL(garner_u32_kernel, grid, block, 0, stream, (const uint32_t*)r0, (const uint32_t*)r1,
+                           (const uint32_t*)r2, N, g, fp[0], fp[1], fp[2], o1, o0);
+    return hipGetLastError();
+}

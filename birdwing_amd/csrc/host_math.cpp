@@ -1,0 +1,268 @@
+// host_math.cpp -- host-side number theory for plans (see host_math.h).
+#include "host_math.h"
+#include <cstring>
+
+namespace bw {
+
+// u32 reference set and u64 reference set: src/gpu_ntt.cu:28-47 (the hybrid
+// set is the first two u64 primes, :40-43).  The wide u32 set {15*2^27+1,
+// 27*2^26+1, 7*2^26+1} (tests/test_crt.cpp:19-21) reaches N = 2^26; its roots
+// are g^((p-1)/2^v2) for the primitive roots g = 31, 13, 3.
+static const PrimeSet kSets[SET_COUNT] = {
+    {SET_U32_REF, 32, 3, {0x2d000001ULL, 0x23800001ULL, 0x26800001ULL}, {663, 721, 19}},
+    {SET_U32_WIDE, 32, 3, {2013265921ULL, 1811939329ULL, 469762049ULL}, {440564289ULL, 72705542ULL, 2187ULL}},
+    {SET_U64_REF, 64, 3, {0x400002600000001ULL, 0x400004200000001ULL, 0x400001100000001ULL},
+     {273765203699653965ULL, 26231613454922890ULL, 144261359744151243ULL}},
+    {SET_HYBRID, 64, 2, {0x400002600000001ULL, 0x400004200000001ULL, 0},
+     {273765203699653965ULL, 26231613454922890ULL, 0}},
+};
+
+const PrimeSet* prime_set(int id) { return (id >= 0 && id < SET_COUNT) ? &kSets[id] : nullptr; }
+
+int default_set(int limb_bits, uint64_t N) {
+    if (limb_bits == 64) return SET_U64_REF;
+    return N <= (1ULL << 23) ? SET_U32_REF : SET_U32_WIDE;
+}
+
+uint64_t mulmod(uint64_t a, uint64_t b, uint64_t p) { return (uint64_t)(((u128)a * b) % p); }
+
+uint64_t powmod(uint64_t a, uint64_t e, uint64_t p) {
+    uint64_t r = 1 % p;
+    a %= p;
+    while (e) {
+        if (e & 1) r = mulmod(r, a, p);
+        a = mulmod(a, a, p);
+        e >>= 1;
+    }
+    return r;
+}
+
+uint64_t invmod(uint64_t a, uint64_t p) { return powmod(a, p - 2, p); }
+
+int valuation2(uint64_t n) {
+    if (n == 0) return 64;
+    int k = 0;
+    while (!(n & 1)) { n >>= 1; k++; }
+    return k;
+}
+
+int max_root_log(uint64_t root, uint64_t p) {   // src/ntt_limits.cpp:37-51
+    uint64_t r = root % p;
+    if (r == 0) return 0;
+    for (int k = 0; k < 62; k++) {
+        if (r == 1) return k;
+        r = mulmod(r, r, p);
+    }
+    return 62;
+}
+
+int set_max_log(const PrimeSet& s) {
+    int lim = 1 << 30;
+    for (int i = 0; i < s.k; i++) {
+        int a = valuation2(s.p[i] - 1), b = max_root_log(s.root_max[i], s.p[i]);
+        int m = a < b ? a : b;
+        if (m < lim) lim = m;
+    }
+    return lim;
+}
+
+uint64_t root_of_order(const PrimeSet& s, int i, int logN) {
+    const uint64_t p = s.p[i];
+    int a = valuation2(p - 1), b = max_root_log(s.root_max[i], p);
+    int maxlog = a < b ? a : b;
+    uint64_t w = s.root_max[i] % p;
+    for (int j = 0; j < maxlog - logN; j++) w = mulmod(w, w, p);
+    return w;
+}
+
+uint64_t padded_ntt_size(uint64_t L_A, uint64_t L_B) {
+    uint64_t L_C = L_A + L_B - 1, N = 1;
+    while (N < L_C) N <<= 1;
+    return N;
+}
+
+bool ntt_size_supported(const PrimeSet& s, uint64_t N, std::string* why) {
+    if (N == 0) { if (why) *why = "NTT size N must be positive"; return false; }
+    if (N & (N - 1)) { if (why) *why = "NTT size N=" + std::to_string(N) + " is not a power of two"; return false; }
+    const int logN = valuation2(N);
+    for (int i = 0; i < s.k; i++) {
+        int a = valuation2(s.p[i] - 1), b = max_root_log(s.root_max[i], s.p[i]);
+        int m = a < b ? a : b;
+        if (logN > m) {
+            if (why)
+                *why = "NTT size N=2^" + std::to_string(logN) + " exceeds modulus " + std::to_string(i) +
+                       " (p=" + std::to_string(s.p[i]) + ", max 2^" + std::to_string(m) + ")";
+            return false;
+        }
+    }
+    return true;
+}
+
+// prod(p) and min(L)*(2^b-1)^2 compared exactly in 256-bit arithmetic.
+struct U256 { uint64_t w[4]; };
+static void mul64(U256& x, uint64_t m) {
+    u128 c = 0;
+    for (int i = 0; i < 4; i++) { c += (u128)x.w[i] * m; x.w[i] = (uint64_t)c; c >>= 64; }
+}
+static bool lt(const U256& a, const U256& b) {
+    for (int i = 3; i >= 0; i--) if (a.w[i] != b.w[i]) return a.w[i] < b.w[i];
+    return false;
+}
+static U256 coeff_bound(int in_bits, uint64_t terms) {
+    U256 x = {{terms, 0, 0, 0}};
+    const uint64_t lm = in_bits == 64 ? ~0ULL : 0xFFFFFFFFULL;
+    mul64(x, lm);
+    mul64(x, lm);
+    return x;
+}
+static U256 modulus_product(const PrimeSet& s) {
+    U256 M = {{1, 0, 0, 0}};
+    for (int i = 0; i < s.k; i++) mul64(M, s.p[i]);
+    return M;
+}
+
+bool crt_bound_ok(const PrimeSet& s, int in_bits, uint64_t L_A, uint64_t L_B, std::string* why) {
+    if (L_A == 0 || L_B == 0) { if (why) *why = "operand limb count must be positive"; return false; }
+    const uint64_t terms = L_A < L_B ? L_A : L_B;
+    if (lt(coeff_bound(in_bits, terms), modulus_product(s))) return true;
+    if (why)
+        *why = "max convolution coefficient for L_A=" + std::to_string(L_A) + ", L_B=" + std::to_string(L_B) +
+               " (terms=" + std::to_string(terms) + ", limb=" + std::to_string(in_bits) +
+               "-bit) exceeds CRT modulus product";
+    return false;
+}
+
+bool multiply_size_supported(const PrimeSet& s, int in_bits, uint64_t L_A, uint64_t L_B, std::string* why) {
+    if (L_A == 0 || L_B == 0) { if (why) *why = "operand limb count must be positive"; return false; }
+    if (!ntt_size_supported(s, padded_ntt_size(L_A, L_B), why)) return false;
+    return crt_bound_ok(s, in_bits, L_A, L_B, why);
+}
+
+uint64_t max_limb_count_by_crt(const PrimeSet& s, int in_bits) {
+    uint64_t lo = 0, hi = ~0ULL;
+    const U256 M = modulus_product(s);
+    while (lo < hi) {
+        uint64_t mid = lo + (hi - lo + 1) / 2;
+        if (lt(coeff_bound(in_bits, mid), M)) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+uint64_t max_supported_limb_count(const PrimeSet& s, int in_bits) {
+    const int ml = set_max_log(s);
+    if (ml <= 0) return 0;
+    const uint64_t by_ntt = 1ULL << (ml - 1), by_crt = max_limb_count_by_crt(s, in_bits);
+    return by_ntt < by_crt ? by_ntt : by_crt;
+}
+
+// ---------------------------------------------------------------------------
+void f64_prime(uint64_t p, F64Prime& out) {
+    out.p = (double)p;
+    out.pinv = 1.0 / (double)p;
+    out.cm = (double)(p - 1);
+}
+
+static void put_f64(std::vector<unsigned char>& v, size_t idx, uint64_t w, uint64_t p) {
+    double pair[2] = {(double)w, (double)w / (double)p};
+    std::memcpy(v.data() + idx * 16, pair, 16);
+}
+static void put_u64(std::vector<unsigned char>& v, size_t idx, uint64_t w, uint64_t p) {
+    uint64_t pair[2] = {w, (uint64_t)(((u128)w << 64) / p)};
+    std::memcpy(v.data() + idx * 16, pair, 16);
+}
+
+void build_tables(const PrimeSet& s, HostTables& out) {
+    const int L = set_max_log(s);
+    out.table_log = L;
+    const size_t nstage = 1u << BW_STAGE_LOG, nlo = 1u << BW_TW_LO_LOG, nhi = (size_t)1 << (L - BW_TW_LO_LOG);
+    out.tw_stage.assign((size_t)s.k * nstage * 16, 0);
+    out.tw_lo.assign((size_t)s.k * nlo * 16, 0);
+    out.tw_hi.assign((size_t)s.k * nhi * 16, 0);
+    const bool f64 = s.limb_bits == 32;
+    out.consts.assign((size_t)s.k * (f64 ? sizeof(F64PrimeDev) : sizeof(U64PrimeDev)), 0);
+    auto put = f64 ? put_f64 : put_u64;
+    for (int i = 0; i < s.k; i++) {
+        const uint64_t p = s.p[i];
+        const uint64_t wL = root_of_order(s, i, L);
+        // stage table: omega_{2^13}^e
+        const uint64_t w13 = powmod(wL, 1ULL << (L - BW_STAGE_LOG), p);
+        uint64_t x = 1;
+        for (size_t e = 0; e < nstage; e++) { put(out.tw_stage, i * nstage + e, x, p); x = mulmod(x, w13, p); }
+        x = 1;
+        for (size_t e = 0; e < nlo; e++) { put(out.tw_lo, i * nlo + e, x, p); x = mulmod(x, wL, p); }
+        const uint64_t wstep = powmod(wL, nlo, p);
+        x = 1;
+        for (size_t e = 0; e < nhi; e++) { put(out.tw_hi, i * nhi + e, x, p); x = mulmod(x, wstep, p); }
+        const uint64_t w16 = powmod(wL, 1ULL << (L - 4), p), iw16 = invmod(w16, p);
+        if (f64) {
+            F64PrimeDev c;
+            std::memset(&c, 0, sizeof(c));
+            f64_prime(p, c.P);
+            uint64_t a = 1, b = 1;
+            for (int e = 0; e < 8; e++) {
+                c.w16[e][0] = (double)a; c.w16[e][1] = (double)a / (double)p;
+                c.iw16[e][0] = (double)b; c.iw16[e][1] = (double)b / (double)p;
+                a = mulmod(a, w16, p); b = mulmod(b, iw16, p);
+            }
+            std::memcpy(out.consts.data() + i * sizeof(F64PrimeDev), &c, sizeof(c));
+        } else {
+            U64PrimeDev c;
+            std::memset(&c, 0, sizeof(c));
+            c.P.p = p;
+            uint64_t inv = 1;                       // Newton: p^-1 mod 2^64
+            for (int it = 0; it < 7; it++) inv *= 2 - p * inv;
+            c.P.pinv_neg = (uint64_t)0 - inv;
+            c.P.r2 = (uint64_t)((((u128)1 << 64) % p) * (u128)(((u128)1 << 64) % p) % p);
+            c.P.floor_r = (uint64_t)(~0ULL / p);
+            uint64_t a = 1, b = 1;
+            for (int e = 0; e < 8; e++) {
+                c.w16[e][0] = a; c.w16[e][1] = (uint64_t)(((u128)a << 64) / p);
+                c.iw16[e][0] = b; c.iw16[e][1] = (uint64_t)(((u128)b << 64) / p);
+                a = mulmod(a, w16, p); b = mulmod(b, iw16, p);
+            }
+            std::memcpy(out.consts.data() + i * sizeof(U64PrimeDev), &c, sizeof(c));
+        }
+    }
+}
+
+GarnerDev garner_constants(const PrimeSet& s) {
+    GarnerDev g;
+    std::memset(&g, 0, sizeof(g));
+    g.k = s.k;
+    for (int i = 0; i < s.k; i++) g.p[i] = s.p[i];
+    g.p0 = s.p[0];
+    const u128 p01 = (u128)s.p[0] * s.p[1];
+    g.p01_lo = (uint64_t)p01;
+    g.p01_hi = (uint64_t)(p01 >> 64);
+    const uint64_t p1 = s.p[1];
+    const uint64_t inv1 = invmod(s.p[0] % p1, p1);
+    g.inv1[0] = (double)inv1; g.inv1[1] = (double)inv1 / (double)p1;
+    g.inv1_s[0] = inv1; g.inv1_s[1] = (uint64_t)(((u128)inv1 << 64) / p1);
+    if (s.k > 2) {
+        const uint64_t p2 = s.p[2];
+        const uint64_t p0m = s.p[0] % p2;
+        const uint64_t inv2 = invmod(mulmod(p0m, s.p[1] % p2, p2), p2);
+        g.inv2[0] = (double)inv2; g.inv2[1] = (double)inv2 / (double)p2;
+        g.inv2_s[0] = inv2; g.inv2_s[1] = (uint64_t)(((u128)inv2 << 64) / p2);
+        g.p0mod2[0] = (double)p0m; g.p0mod2[1] = (double)p0m / (double)p2;
+        g.p0mod2_s[0] = p0m; g.p0mod2_s[1] = (uint64_t)(((u128)p0m << 64) / p2);
+    }
+    return g;
+}
+
+void ninv_pairs(const PrimeSet& s, uint64_t N, std::vector<uint64_t>& out64, std::vector<double>& outf) {
+    out64.assign(2 * s.k, 0);
+    outf.assign(2 * s.k, 0.0);
+    for (int i = 0; i < s.k; i++) {
+        const uint64_t p = s.p[i];
+        const uint64_t ni = invmod(N % p, p);
+        outf[2 * i] = (double)ni;
+        outf[2 * i + 1] = (double)ni / (double)p;
+        // Montgomery: montmul(montmul(a, b), K) = a b N^-1 needs K = N^-1 2^128 mod p
+        const uint64_t r = (uint64_t)((((u128)1 << 64)) % p);
+        out64[2 * i] = mulmod(mulmod(ni, r, p), r, p);
+        out64[2 * i + 1] = 0;
+    }
+}
+
+}  // namespace bw

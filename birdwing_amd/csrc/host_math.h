@@ -1,0 +1,56 @@
+// host_math.h -- host-side number theory for plans: prime sets, roots,
+// twiddle tables, Garner constants, size policy (src/ntt_limits.cpp).
+#pragma once
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "bw_internal.h"
+
+namespace bw {
+
+enum { SET_U32_REF = 0, SET_U32_WIDE = 1, SET_U64_REF = 2, SET_HYBRID = 3, SET_COUNT = 4 };
+
+struct PrimeSet {
+    int id;
+    int limb_bits;       // arithmetic: 32 -> FP64 kernels, 64 -> integer kernels
+    int k;
+    uint64_t p[3];
+    uint64_t root_max[3];
+};
+
+const PrimeSet* prime_set(int id);
+int default_set(int limb_bits, uint64_t N);
+
+typedef unsigned __int128 u128;
+uint64_t mulmod(uint64_t a, uint64_t b, uint64_t p);
+uint64_t powmod(uint64_t a, uint64_t e, uint64_t p);
+uint64_t invmod(uint64_t a, uint64_t p);   // p prime
+
+int valuation2(uint64_t n);
+int max_root_log(uint64_t root, uint64_t p);
+int set_max_log(const PrimeSet& s);
+// root of order 2^logN (src/gpu_ntt.cu:171-192: square root_max down from the max order)
+uint64_t root_of_order(const PrimeSet& s, int i, int logN);
+
+uint64_t padded_ntt_size(uint64_t L_A, uint64_t L_B);
+bool ntt_size_supported(const PrimeSet& s, uint64_t N, std::string* why);
+bool crt_bound_ok(const PrimeSet& s, int in_bits, uint64_t L_A, uint64_t L_B, std::string* why);
+bool multiply_size_supported(const PrimeSet& s, int in_bits, uint64_t L_A, uint64_t L_B, std::string* why);
+uint64_t max_limb_count_by_crt(const PrimeSet& s, int in_bits);
+uint64_t max_supported_limb_count(const PrimeSet& s, int in_bits);
+
+// Host images of the device tables.
+struct HostTables {
+    int table_log;
+    std::vector<unsigned char> consts;    // F64PrimeDev[k] / U64PrimeDev[k]
+    std::vector<unsigned char> tw_stage;  // k * 2^13 pairs (16 B)
+    std::vector<unsigned char> tw_lo;     // k * 2^12 pairs
+    std::vector<unsigned char> tw_hi;     // k * 2^(L-12) pairs
+};
+void build_tables(const PrimeSet& s, HostTables& out);
+GarnerDev garner_constants(const PrimeSet& s);
+void f64_prime(uint64_t p, F64Prime& out);
+// Per-plan (N^-1, aux) pairs, layout [k][2].
+void ninv_pairs(const PrimeSet& s, uint64_t N, std::vector<uint64_t>& out64, std::vector<double>& outf);
+
+}  // namespace bw

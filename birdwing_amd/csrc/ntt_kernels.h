@@ -1,0 +1,59 @@
+// ntt_kernels.h -- multi-pass number-theoretic transform kernels for gfx950.
+//
+// A length-N transform is split into passes of T_j = 2^logT points (four-step
+// decomposition applied recursively).  Pass j views the N-array as
+// (O_j, T_j, S_j) and, for each of the O_j*S_j "lines" (o, s), computes
+//
+//   forward:  y[k] = sum_t x[(o,t,s)] w_T^{t k};   out[(o,q(k),s)] = y[k] * w_M^{k s}
+//   inverse:  z[k] = in[(o,q(k),s)] * w_M^{-k s};  out[(o,t,s)] = sum_k z[k] w_T^{-t k}
+//
+// with M = T*S.  Inside a workgroup the line is staged through LDS and the
+// T-point transform is done as radix-16 register codelets (decimation in
+// frequency forward, decimation in time inverse), so the forward output lands
+// at the digit-reversed position q(k) and the inverse consumes that order: no
+// permutation pass ever runs.  The final layout of the spectrum is therefore a
+// fixed permutation of natural order (pointwise products do not care).
+#pragma once
+#include "bw_internal.h"
+
+// Radix schedule of a 2^LOGT-point sub-transform: stage 0 has radix
+// 2^(LOGT mod 4) (16 when LOGT is a multiple of 4), every later stage 16.
+__host__ __device__ constexpr int sched_r0log(int logT) { return logT % 4 == 0 ? 4 : logT % 4; }
+__host__ __device__ constexpr int sched_nst(int logT) { return logT == 0 ? 0 : (logT - sched_r0log(logT)) / 4 + 1; }
+__host__ __device__ constexpr int sched_rlog(int logT, int i) { return i == 0 ? sched_r0log(logT) : 4; }
+// log2 of the span (block size) that stage i works on
+__host__ __device__ constexpr int sched_spanlog(int logT, int i) { return i == 0 ? logT : logT - sched_r0log(logT) - 4 * (i - 1); }
+
+// Frequency index of the value a forward pass leaves at position t of a line.
+__host__ __device__ inline int64_t sched_freq_of_pos(int logT, int64_t t) {
+    int64_t f = 0;
+    int shift = 0;
+    const int nst = sched_nst(logT);
+    for (int i = 0; i < nst; i++) {
+        const int rl = sched_rlog(logT, i);
+        const int stridelog = sched_spanlog(logT, i) - rl;
+        const int64_t k = (t >> stridelog) & ((1 << rl) - 1);
+        f |= k << shift;
+        shift += rl;
+    }
+    return f;
+}
+
+// LDS padding: one slot per 16 elements breaks the stride-16 bank pattern of
+// the last stage.
+__host__ __device__ constexpr int lds_pad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int lds_line(int logT) { return (1 << logT) + ((1 << logT) >> 4); }
+
+// Launch one pass.  Returns hipSuccess or the launch error.
+struct PassLaunch {
+    int mode;          // BW_MODE_*
+    int logT;
+    int G;
+    int limb_bits;     // 32 -> FP64 arithmetic, 64 -> integer arithmetic
+    int nprimes;       // grid.y
+    int nops;          // grid.z (FWD)
+    int64_t tiles;     // grid.x
+};
+hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_t stream);
+// Largest lines-per-workgroup the kernel set provides for a pass of 2^logT points.
+int bw_pass_max_G(int logT);

@@ -1,0 +1,154 @@
+/*
+ * birdwing.h -- C ABI of libbirdwing_hip.so, the MI355X (gfx950) big-integer
+ * multiplier.  Plain pointers and sizes only; every call returns a status
+ * code (BW_OK == 0) instead of calling exit(), and bw_last_error() returns the
+ * message of the calling thread's last failure.
+ *
+ * Product method (the reference's, src/gpu_ntt.cu:442-973): zero-pad the
+ * little-endian limbs of A and B to N = next_pow2(L_A + L_B - 1), forward NTT
+ * under each RNS prime, pointwise product, inverse NTT, Garner CRT, carry
+ * propagation.  C has L_A + L_B limbs (src/host_multiply.cpp:30,47).
+ *
+ * Digit widths (a runtime argument here; compile-time LIMB_BITS in the
+ * reference, include/config.h:17-53):
+ *   limb_bits = 32: 32-bit limbs, three ~30-bit primes, 128-bit Garner.
+ *   limb_bits = 64: 64-bit limbs, three 58-bit primes, 160-bit Garner.
+ * Prime sets (src/gpu_ntt.cu:28-47 plus one wide 32-bit set):
+ *   BW_SET_U32_REF  {0x2d000001, 0x23800001, 0x26800001}, N <= 2^23
+ *   BW_SET_U32_WIDE {2013265921, 1811939329, 469762049},  N <= 2^26
+ *   BW_SET_U64_REF  {0x400002600000001, 0x400004200000001, 0x400001100000001}
+ *   BW_SET_HYBRID   first two u64 primes (u32 limbs; reserved)
+ * Passing prime_set = -1 picks BW_SET_U32_REF while N <= 2^23, BW_SET_U32_WIDE
+ * above (u32), BW_SET_U64_REF (u64).
+ *
+ * Streams are hipStream_t values passed as void* (NULL = default stream).
+ * Device entry points are asynchronous on that stream; *_host entry points
+ * synchronize before returning.  Plans are independent objects: distinct plans
+ * may be used from distinct threads; one plan must not run two executions at
+ * once (they share scratch buffers).
+ */
+#ifndef BIRDWING_H
+#define BIRDWING_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    BW_OK = 0,
+    BW_ERR_ARG = -1,
+    BW_ERR_UNSUPPORTED_SIZE = -2,
+    BW_ERR_HIP = -3,
+    BW_ERR_NOMEM = -4,
+    BW_ERR_STATE = -5
+};
+
+enum { BW_SET_DEFAULT = -1, BW_SET_U32_REF = 0, BW_SET_U32_WIDE = 1, BW_SET_U64_REF = 2, BW_SET_HYBRID = 3 };
+
+typedef struct bw_plan bw_plan;
+
+typedef struct bw_plan_info {
+    int limb_bits;
+    int prime_set;
+    int k;                 /* number of primes */
+    uint64_t primes[3];
+    uint64_t L_A, L_B;
+    uint64_t N;            /* next_pow2(L_A + L_B - 1), the reference's padded size */
+    uint64_t N_exec;       /* transform length actually run (>= 2) */
+    int npass;             /* passes per transform */
+    int pass_log[4];       /* log2 of each pass's points */
+    int nlaunch;           /* kernel launches per bw_execute */
+    uint64_t device_bytes; /* device memory held by the plan */
+} bw_plan_info;
+
+/* ---- library ---------------------------------------------------------- */
+const char* bw_last_error(void);
+const char* bw_version(void);
+/* hipSetDevice + table warm-up for the device; optional. */
+int bw_init(int device);
+
+/* ---- size policy (src/ntt_limits.cpp) ---------------------------------- */
+uint64_t bw_padded_ntt_size(uint64_t L_A, uint64_t L_B);                         /* :145-152 */
+int bw_default_prime_set(int limb_bits, uint64_t N);
+int bw_prime_set_info(int prime_set, int* k, uint64_t* primes, uint64_t* roots);  /* src/gpu_ntt.cu:28-47 */
+int bw_max_supported_logN(int prime_set);                                         /* :125-135 */
+uint64_t bw_max_limb_count_by_crt(int prime_set, int in_bits);                    /* :154-174 */
+uint64_t bw_max_supported_limb_count(int prime_set, int in_bits);                 /* :176-184 */
+/* The three predicates return 1/0 and fill `why` (if non-NULL) on rejection. */
+int bw_ntt_size_supported(int prime_set, uint64_t N, char* why, size_t why_len);  /* :216-239 */
+int bw_crt_bound_satisfied(int prime_set, int in_bits, uint64_t L_A, uint64_t L_B,
+                           char* why, size_t why_len);                              /* :186-214 */
+int bw_multiply_size_supported(int prime_set, int in_bits, uint64_t L_A, uint64_t L_B,
+                               char* why, size_t why_len);                          /* :241-254 */
+
+/* ---- plans: precompute_ntt + upload_ntt_precomputed + allocate_ntt_context
+ *      (src/gpu_ntt.cu:240-440), cleanup_ntt_context + cleanup_ntt_precomputed
+ *      (:975-1019).  Twiddle tables are built once per prime set and device and
+ *      shared by all plans. */
+int bw_plan_create(bw_plan** plan, int limb_bits, uint64_t L_A, uint64_t L_B, int prime_set);
+/* Same, with an explicit pass split (sum of pass_log == log2 N_exec, each <= 13). */
+int bw_plan_create_ex(bw_plan** plan, int limb_bits, uint64_t L_A, uint64_t L_B, int prime_set,
+                      const int* pass_log, int npass);
+void bw_plan_destroy(bw_plan* plan);
+int bw_plan_get_info(const bw_plan* plan, bw_plan_info* info);
+
+/* ---- execution: execute_ntt_multiply (src/gpu_ntt.cu:442-973) --------------
+ * d_a: L_A limbs, d_b: L_B limbs, d_c: L_A + L_B limbs, all device memory.
+ * Runs on `stream` with no host synchronization. */
+int bw_execute(bw_plan* plan, const void* d_a, const void* d_b, void* d_c, void* stream);
+/* Early exit after Garner (src/gpu_ntt.cu:843-884): N_exec coefficients as
+ * 32-bit path: c0 = lo 64 bits, c1 = hi 64 bits (c2 unused, may be NULL);
+ * 64-bit path: c0 = lo, c1 = mid, c2 = hi 32 bits (U160). */
+int bw_execute_crt(bw_plan* plan, const void* d_a, const void* d_b, uint64_t* d_c0, uint64_t* d_c1,
+                   uint32_t* d_c2, void* stream);
+/* Synchronizes and reports BW_ERR_STATE if a carry look-back wait of this plan
+ * ever exceeded its bound (a failure that would otherwise be a hang). */
+int bw_plan_status(bw_plan* plan);
+/* Host buffers; copies in, runs, copies out, synchronizes. */
+int bw_execute_host(bw_plan* plan, const void* h_a, const void* h_b, void* h_c);
+/* Per-launch timing of the next executions (hipEvents on the plan's stream);
+ * bw_plan_timing returns the durations (ms) of the last execution's launches. */
+int bw_plan_set_timing(bw_plan* plan, int enabled);
+int bw_plan_timing(const bw_plan* plan, float* ms, int max, int* n);
+
+/* ---- one-shot host multiply: host_multiply_merge / host_multiply_merge_native
+ *      (src/host_multiply.cpp:27-134).  c gets L_A + L_B limbs; *ms (optional)
+ *      gets the wall time from plan creation to plan release. */
+int bw_multiply_host(int limb_bits, const void* h_a, uint64_t L_A, const void* h_b, uint64_t L_B, void* h_c,
+                     double* ms);
+
+/* ---- stage-level entry points ------------------------------------------- */
+/* Forward transforms of raw limbs (L of them, zero-padded to N_exec) under every
+ * prime of the plan: d_spec gets k planes of N_exec words (u32 words on the
+ * 32-bit path, u64 on the 64-bit path) in the plan's spectrum order. */
+int bw_forward(bw_plan* plan, const void* d_raw, uint64_t L, void* d_spec, void* stream);
+/* Inverse transforms in place (k planes), including the N^-1 scale. */
+int bw_inverse(bw_plan* plan, void* d_spec, void* stream);
+/* order[pos] = frequency index held at spectrum position pos (N_exec entries). */
+int bw_spectrum_order(const bw_plan* plan, uint64_t* order);
+/* src/zero_pad.cu:14-24 -- widen 32-bit limbs to word_bytes (4 or 8), no reduction. */
+int bw_zero_pad(const uint32_t* d_src, void* d_dst, int word_bytes, uint64_t L, uint64_t N, void* stream);
+/* src/zero_pad.cu:44-56 -- 64-bit limbs reduced mod p. */
+int bw_zero_pad_u64(const uint64_t* d_src, uint64_t* d_dst, uint64_t L, uint64_t N, uint64_t p, void* stream);
+/* src/gpu_ntt.cu:216-238 -- c = a*b mod p elementwise (word_bytes 4 or 8). */
+int bw_pointwise(int word_bytes, const void* d_a, const void* d_b, void* d_c, uint64_t N, uint64_t p,
+                 void* stream);
+/* src/crt_gpu.cu:278-302 -- Garner over k = 3 residue arrays of the prime set.
+ * 32-bit sets: o0 = lo, o1 = hi (128-bit).  64-bit set: o0 = lo, o1 = mid, o2 = hi32. */
+int bw_crt_combine(int prime_set, const void* d_r0, const void* d_r1, const void* d_r2, uint64_t N,
+                   uint64_t* d_o0, uint64_t* d_o1, uint32_t* d_o2, void* stream);
+/* src/carry_prop.cu (3-pass scheme) replaced by a single-pass carry scan.
+ * 128-bit coefficients (hi, lo) -> n_out limbs of out_bits (32 or 64);
+ * coefficients beyond N count as zero. */
+int bw_carry128(const uint64_t* d_hi, const uint64_t* d_lo, uint64_t N, void* d_out, uint64_t n_out,
+                int out_bits, void* stream);
+/* U160 coefficients (lo, mid, hi32) -> n_out 64-bit limbs. */
+int bw_carry160(const uint64_t* d_lo, const uint64_t* d_mid, const uint32_t* d_hi, uint64_t N, void* d_out,
+                uint64_t n_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
