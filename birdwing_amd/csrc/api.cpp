@@ -191,10 +191,11 @@ struct bw_plan {
     void* h_b = nullptr;
     void* h_c = nullptr;
     hipStream_t own = nullptr;
-    // timing
+    // timing: one event set (launches + 1 events) per recorded execution
     bool timing = false;
-    hipEvent_t ev[BW_MAX_LAUNCH + 1] = {};
-    int nev = 0;
+    std::vector<std::vector<hipEvent_t>> evpool;
+    size_t nexec = 0;          // executions recorded since timing was enabled
+    int nev = 0;               // events recorded by the current execution
 };
 
 static void default_passes(int logN, std::vector<int>& out) {
@@ -224,8 +225,8 @@ static void plan_free(bw_plan* p) {
     (void)hipFree(p->h_b);
     (void)hipFree(p->h_c);
     if (p->own) (void)hipStreamDestroy(p->own);
-    for (int i = 0; i <= BW_MAX_LAUNCH; i++)
-        if (p->ev[i]) (void)hipEventDestroy(p->ev[i]);
+    for (auto& set : p->evpool)
+        for (auto e : set) (void)hipEventDestroy(e);
     delete p;
 }
 
@@ -400,9 +401,19 @@ struct Marker {
     hipStream_t st;
     int n = 0;
     hipError_t mark() {
-        if (!p->timing) return hipSuccess;
-        if (n > BW_MAX_LAUNCH) return hipSuccess;
-        hipError_t e = hipEventRecord(p->ev[n], st);
+        if (!p->timing || n > BW_MAX_LAUNCH) return hipSuccess;
+        if (n == 0) {                                  // first mark of an execution: take an event set
+            if (p->nexec == p->evpool.size()) {
+                std::vector<hipEvent_t> set(BW_MAX_LAUNCH + 1);
+                for (auto& e : set) {
+                    hipError_t r = hipEventCreate(&e);
+                    if (r != hipSuccess) return r;
+                }
+                p->evpool.push_back(set);
+            }
+            p->nexec++;
+        }
+        hipError_t e = hipEventRecord(p->evpool[p->nexec - 1][n], st);
         n++;
         p->nev = n;
         return e;
@@ -558,9 +569,9 @@ extern "C" int bw_plan_status(bw_plan* p) {
 extern "C" int bw_plan_set_timing(bw_plan* p, int enabled) {
     int rc = check_plan(p);
     if (rc) return rc;
-    if (enabled && !p->ev[0])
-        for (int i = 0; i <= BW_MAX_LAUNCH; i++) HIP_TRY(hipEventCreate(&p->ev[i]));
     p->timing = enabled != 0;
+    p->nexec = 0;                 // start a new recording (event sets are reused)
+    p->nev = 0;
     return BW_OK;
 }
 
@@ -569,9 +580,17 @@ extern "C" int bw_plan_timing(const bw_plan* p, float* ms, int max, int* n) {
     if (rc) return rc;
     const int cnt = p->nev > 0 ? p->nev - 1 : 0;
     if (n) *n = cnt;
-    if (cnt == 0) return BW_OK;
-    HIP_TRY(hipEventSynchronize(p->ev[cnt]));
-    for (int i = 0; i < cnt && i < max; i++) HIP_TRY(hipEventElapsedTime(&ms[i], p->ev[i], p->ev[i + 1]));
+    if (cnt == 0 || p->nexec == 0) return BW_OK;
+    for (int i = 0; i < cnt && i < max; i++) ms[i] = 0.0f;
+    for (size_t x = 0; x < p->nexec; x++) {
+        const auto& set = p->evpool[x];
+        HIP_TRY(hipEventSynchronize(set[cnt]));
+        for (int i = 0; i < cnt && i < max; i++) {
+            float t = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&t, set[i], set[i + 1]));
+            ms[i] += t / (float)p->nexec;
+        }
+    }
     return BW_OK;
 }
 

@@ -139,7 +139,7 @@ bool multiply_size_supported(const PrimeSet& s, int in_bits, uint64_t L_A, uint6
 }
 
 uint64_t max_limb_count_by_crt(const PrimeSet& s, int in_bits) {
-    uint64_t lo = 0, hi = ~0ULL;
+    uint64_t lo = 0, hi = 1ULL << 63;     // every set's bound is below 2^63
     const U256 M = modulus_product(s);
     while (lo < hi) {
         uint64_t mid = lo + (hi - lo + 1) / 2;

@@ -108,8 +108,10 @@ int bw_execute_crt(bw_plan* plan, const void* d_a, const void* d_b, uint64_t* d_
 int bw_plan_status(bw_plan* plan);
 /* Host buffers; copies in, runs, copies out, synchronizes. */
 int bw_execute_host(bw_plan* plan, const void* h_a, const void* h_b, void* h_c);
-/* Per-launch timing of the next executions (hipEvents on the plan's stream);
- * bw_plan_timing returns the durations (ms) of the last execution's launches. */
+/* Per-launch timing (hipEvents recorded on the execution's stream between
+ * launches).  bw_plan_set_timing(plan, 1) starts a recording; bw_plan_timing
+ * synchronizes and returns, for each launch of an execution, its duration in ms
+ * averaged over every execution recorded since then. */
 int bw_plan_set_timing(bw_plan* plan, int enabled);
 int bw_plan_timing(const bw_plan* plan, float* ms, int max, int* n);
 

@@ -210,8 +210,9 @@ int orc_multiply_size_supported(int set, int in_bits, uint64_t L_A, uint64_t L_B
 }
 
 uint64_t orc_max_limb_count_by_crt(int set, int in_bits) {
-    /* largest L with L*(2^b-1)^2 < M, i.e. floor((M-1)/(2^b-1)^2): binary search */
-    uint64_t lo = 0, hi = UINT64_MAX;
+    /* largest L with L*(2^b-1)^2 < M, i.e. floor((M-1)/(2^b-1)^2): binary search
+     * (every prime set here gives a bound below 2^63) */
+    uint64_t lo = 0, hi = 1ULL << 63;
     while (lo < hi) {
         uint64_t mid = lo + (hi - lo + 1) / 2;
         u256 c = limb_sq_times(in_bits, mid), M = prime_prod(set);
@@ -312,6 +313,58 @@ void orc_dft_bruteforce(const uint64_t* x, uint64_t* X, uint64_t N, uint64_t p, 
             wnk = orc_mulmod(wnk, wk, p);
         }
         X[k] = acc;
+    }
+}
+
+/* Pipeline transforms without permutations: decimation in frequency forward
+ * (natural order in, bit-reversed order out) and decimation in time inverse
+ * (bit-reversed in, natural out, times N^-1).  The pointwise product between
+ * them does not depend on the order.  Twiddles are stored per level,
+ * tw[m + j] = w_{2m}^j for j < m, so every level reads them contiguously. */
+typedef struct { uint64_t* w; uint64_t* q; } orc_tw;
+
+static orc_tw tw_levels(uint64_t N, uint64_t p, uint64_t root) {
+    orc_tw t;
+    t.w = (uint64_t*)malloc(sizeof(uint64_t) * (N ? N : 1));
+    t.q = (uint64_t*)malloc(sizeof(uint64_t) * (N ? N : 1));
+    for (uint64_t m = 1; m < N; m <<= 1) {
+        uint64_t wm = orc_powmod(root, N / (2 * m), p), x = 1;
+        for (uint64_t j = 0; j < m; j++) {
+            t.w[m + j] = x;
+            t.q[m + j] = (uint64_t)(((u128)x << 64) / p);
+            x = orc_mulmod(x, wm, p);
+        }
+    }
+    return t;
+}
+
+static void ntt_dif_noperm(uint64_t* x, uint64_t N, uint64_t p, const orc_tw* t) {
+    for (uint64_t m = N / 2; m >= 1; m >>= 1) {
+        for (uint64_t blk = 0; blk < N; blk += 2 * m) {
+            for (uint64_t j = 0; j < m; j++) {
+                uint64_t u = x[blk + j], v = x[blk + j + m];
+                uint64_t s = u + v; if (s >= p) s -= p;
+                uint64_t d = u >= v ? u - v : u + p - v;
+                x[blk + j] = s;
+                x[blk + j + m] = shoup_mul(d, t->w[m + j], t->q[m + j], p);
+            }
+        }
+        if (m == 1) break;
+    }
+}
+
+static void ntt_dit_noperm(uint64_t* x, uint64_t N, uint64_t p, const orc_tw* it) {
+    for (uint64_t m = 1; m < N; m <<= 1) {
+        for (uint64_t blk = 0; blk < N; blk += 2 * m) {
+            for (uint64_t j = 0; j < m; j++) {
+                uint64_t u = x[blk + j];
+                uint64_t v = shoup_mul(x[blk + j + m], it->w[m + j], it->q[m + j], p);
+                uint64_t s = u + v; if (s >= p) s -= p;
+                uint64_t d = u >= v ? u - v : u + p - v;
+                x[blk + j] = s;
+                x[blk + j + m] = d;
+            }
+        }
     }
 }
 
@@ -459,19 +512,25 @@ int orc_multiply_crt(const void* A, uint64_t L_A, const void* B, uint64_t L_B, i
     uint64_t P[3], R[3];
     int k = orc_prime_set(set, P, R);
     uint64_t* res = (uint64_t*)malloc(sizeof(uint64_t) * N * (uint64_t)k);
-    uint64_t* fb = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    /* one prime per thread (the primes are independent until Garner) */
+    #pragma omp parallel for schedule(static, 1)
     for (int j = 0; j < k; j++) {
         uint64_t p = P[j], w = orc_root_for_N(p, R[j], logN);
         uint64_t* fa = res + (uint64_t)j * N;
+        uint64_t* fb = (uint64_t*)malloc(sizeof(uint64_t) * N);
         /* ingress: zero-pad and reduce (src/zero_pad.cu:3-42) */
         for (uint64_t i = 0; i < N; i++) fa[i] = i < L_A ? limb_at(A, i, limb_bits) % p : 0;
         for (uint64_t i = 0; i < N; i++) fb[i] = i < L_B ? limb_at(B, i, limb_bits) % p : 0;
-        orc_ntt_forward(fa, N, p, w);
-        orc_ntt_forward(fb, N, p, w);
-        orc_pointwise(fa, fb, fa, N, p);
-        orc_ntt_inverse(fa, N, p, w);
+        orc_tw tw = tw_levels(N, p, w);
+        orc_tw itw = tw_levels(N, p, orc_modinv(w, p));
+        ntt_dif_noperm(fa, N, p, &tw);
+        ntt_dif_noperm(fb, N, p, &tw);
+        uint64_t ninv = orc_modinv(N % p, p);
+        for (uint64_t i = 0; i < N; i++) fa[i] = orc_mulmod(orc_mulmod(fa[i], fb[i], p), ninv, p);
+        ntt_dit_noperm(fa, N, p, &itw);
+        free(tw.w); free(tw.q); free(itw.w); free(itw.q);
+        free(fb);
     }
-    free(fb);
     orc_garner g; orc_garner_params(&g, P, k);
     uint64_t r[3], x[3];
     for (uint64_t n = 0; n < N; n++) {

@@ -1,0 +1,77 @@
+"""CPU checks of the C ABI library: it loads, exports every entry point that
+include/birdwing.h declares, and its host-only size policy agrees with the
+oracle and the reference's own tests/test_ntt_limits.cpp expectations.  No
+compute calls (no GPU here)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "birdwing.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(bw_\w+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    import birdwing_amd as bw
+    L = bw.lib()
+    names = declared_symbols()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(L, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", bw.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (bw_\w+)", out))
+    assert set(names) <= exported
+    assert set(names) == set(bw.SIGNATURES), "Python bindings out of sync with the header"
+
+
+def test_version_and_error_string():
+    import birdwing_amd as bw
+    assert b"gfx950" in bw.lib().bw_version()
+    assert isinstance(bw.lib().bw_last_error(), bytes)
+
+
+def test_size_policy_matches_oracle(oracle):
+    import birdwing_amd as bw
+    O = oracle.lib()
+    for s in (0, 1, 2, 3):
+        assert bw.max_supported_logN(s) == O.orc_max_supported_logN(s)
+        for bits in (32, 64):
+            assert bw.max_limb_count_by_crt(s, bits) == O.orc_max_limb_count_by_crt(s, bits)
+            assert bw.max_supported_limb_count(s, bits) == O.orc_max_supported_limb_count(s, bits)
+    for L1, L2 in ((1, 1), (4, 4), (5, 5), (1 << 22, 1 << 22), (1 << 23, 1 << 23), (3, 1 << 20)):
+        assert bw.padded_ntt_size(L1, L2) == O.orc_padded_ntt_size(L1, L2)
+        for s, bits in ((0, 32), (1, 32), (2, 64), (3, 32)):
+            assert bw.multiply_size_supported(s, bits, L1, L2)[0] == bool(O.orc_multiply_size_supported(s, bits, L1, L2))
+            assert bw.crt_bound_satisfied(s, bits, L1, L2)[0] == bool(O.orc_crt_bound_ok(s, bits, L1, L2))
+    for logN in range(0, 36):
+        for s in (0, 1, 2, 3):
+            assert bw.ntt_size_supported(s, 1 << logN)[0] == bool(O.orc_ntt_size_supported(s, 1 << logN))
+
+
+def test_rejection_messages():
+    """tests/test_ntt_limits.cpp:186-191,277-281: messages name the failing bound."""
+    import birdwing_amd as bw
+    ok, why = bw.crt_bound_satisfied(0, 32, 1 << 24, 1 << 24)
+    assert not ok and "CRT modulus product" in why
+    ok, why = bw.ntt_size_supported(0, 1 << 24)
+    assert not ok and "24" in why
+    ok, why = bw.ntt_size_supported(0, 3)
+    assert not ok and "power of two" in why
+    ok, why = bw.multiply_size_supported(0, 32, 0, 5)
+    assert not ok and "positive" in why
+
+
+def test_default_prime_sets():
+    import birdwing_amd as bw
+    assert bw.default_prime_set(32, 4096) == 0
+    assert bw.default_prime_set(32, 1 << 23) == 0
+    assert bw.default_prime_set(32, 1 << 25) == 1
+    assert bw.default_prime_set(64, 1 << 25) == 2
+    assert bw.prime_set_info(1)[0] == [2013265921, 1811939329, 469762049]
