@@ -569,9 +569,11 @@ extern "C" int bw_plan_status(bw_plan* p) {
 extern "C" int bw_plan_set_timing(bw_plan* p, int enabled) {
     int rc = check_plan(p);
     if (rc) return rc;
-    p->timing = enabled != 0;
-    p->nexec = 0;                 // start a new recording (event sets are reused)
-    p->nev = 0;
+    if (enabled) {                // start a new recording (event sets are reused)
+        p->nexec = 0;
+        p->nev = 0;
+    }
+    p->timing = enabled != 0;     // disabling keeps the recording readable
     return BW_OK;
 }
 
