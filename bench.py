@@ -44,19 +44,30 @@ def synthetic_limbs(n, seed, bits):
     return a
 
 
-def launch_names(plan):
+def launch_names(plan, schedule):
+    """(name, operand-planes) per launch, in issue order."""
     P = len(plan.passes)
     logs = plan.passes
+    k = plan.k
     if P == 1:
-        return ["fused_pass(T=2^%d)" % logs[0], "garner_carry"]
-    names = ["fwd_pass%d(T=2^%d)" % (j + 1, logs[j]) for j in range(P - 1)]
-    names.append("fused_pass%d(T=2^%d)" % (P, logs[P - 1]))
-    names += ["inv_pass%d(T=2^%d)" % (j + 1, logs[j]) for j in range(P - 2, -1, -1)]
-    names.append("garner_carry")
-    return names
+        return [("fused_pass(T=2^%d)" % logs[0], k), ("garner_carry", k)]
+    out = []
+    chunks = range(k) if schedule == 1 else [None]
+    for c in chunks:
+        tag = "" if c is None else "[p%d]" % c
+        kk = 1 if c is not None else k
+        ops = ["A", "B"] if c is not None else ["AB"]
+        for o in ops:
+            for j in range(P - 1):
+                out.append(("fwd_pass%d(T=2^%d)%s%s" % (j + 1, logs[j], tag, o), kk * (2 if o == "AB" else 1)))
+        out.append(("fused_pass%d(T=2^%d)%s" % (P, logs[P - 1], tag), kk))
+        for j in range(P - 2, -1, -1):
+            out.append(("inv_pass%d(T=2^%d)%s" % (j + 1, logs[j], tag), kk))
+    out.append(("garner_carry", k))
+    return out
 
 
-def launch_bytes(plan, bits):
+def launch_bytes(plan, bits, names):
     """Algorithmic bytes per launch (staged one-pass model).
 
     fwd pass 1 (ingress):  read A,B limbs, write 2k spectra      = S1
@@ -68,18 +79,21 @@ def launch_bytes(plan, bits):
                            128/160-bit coefficient round trip the fusion removes
     """
     w = bits // 8
-    k, N = plan.k, plan.N_exec
+    N = plan.N_exec
     L = plan.L_A + plan.L_B
     P = len(plan.passes)
     out = []
-    if P == 1:
-        out.append(L * w + k * N * w)                # raw limbs in, k product planes out
-    else:
-        out.append(L * w + 2 * k * N * w)
-        out += [4 * k * N * w] * (P - 2)
-        out.append(3 * k * N * w)
-        out += [2 * k * N * w] * (P - 1)
-    out.append(k * N * w + L * w)
+    for name, planes in names:
+        if name.startswith("garner"):
+            out.append(plan.k * N * w + L * w)
+        elif name.startswith("fused"):
+            out.append((L * w + planes * N * w) if P == 1 else 3 * planes * N * w)
+        elif name.startswith("fwd_pass1"):
+            nop = 2 if name.endswith("AB") else 1
+            raw = L * w if nop == 2 else (plan.L_A if name.endswith("A") else plan.L_B) * w
+            out.append(raw + planes * N * w)
+        else:
+            out.append(2 * planes * N * w)
     return out
 
 
@@ -110,8 +124,9 @@ def cpu_baselines(A, B, bits, gpu_out, sample_oracle):
     if sample_oracle:
         ref = pyoracle.multiply(A, B, bits)
         t = time.perf_counter() - t0
-        res["oracle"] = {"value": 1.0 / t, "unit": "products/s", "cores": 1, "kind": "port",
-                         "sample": "one %dx%d-limb product (%d-bit limbs), %.1f s" % (len(A), len(B), bits, t)}
+        res["oracle"] = {"value": 1.0 / t, "unit": "products/s", "cores": 3, "kind": "port",
+                         "sample": "one %dx%d-limb product (%d-bit limbs, one prime per thread), %.1f s"
+                                   % (len(A), len(B), bits, t)}
     exact = None
     if pyoracle.gmp_available():
         g, tg = pyoracle.gmp_mul(A, B, bits)
@@ -133,6 +148,7 @@ def main():
     ap.add_argument("--limbs", type=int, default=1 << 24, help="limbs per operand")
     ap.add_argument("--bits", type=int, default=32, choices=(32, 64))
     ap.add_argument("--passes", type=str, default="", help="pass split, e.g. 12,13")
+    ap.add_argument("--schedule", type=int, default=1, help="0 pass-major, 1 prime-major launches")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-oracle", action="store_true", help="GMP only as CPU baseline")
     args = ap.parse_args()
@@ -155,6 +171,7 @@ def main():
     db = torch.from_numpy(B.view(ndt)).cuda()
     dc = torch.empty(2 * L, dtype=tdt, device="cuda")
     plan = bw.Plan(bits, L, L, bw.SET_DEFAULT, passes)
+    plan.set_schedule(args.schedule)
     stream = torch.cuda.current_stream()
 
     for _ in range(args.warmup):
@@ -180,10 +197,21 @@ def main():
     lat = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps))
     p50 = float(np.percentile(lat, 50))
     p99 = float(np.percentile(lat, 99))
-    names = launch_names(plan)
-    lbytes = launch_bytes(plan, bits)
-    dom = int(np.argmax(per_launch))
-    dom_gbps = lbytes[dom] / (per_launch[dom] * 1e-3) / 1e9
+    nl = launch_names(plan, args.schedule)
+    names = [n for n, _ in nl]
+    lbytes = launch_bytes(plan, bits, nl)
+    # aggregate launches of one kernel type (across primes and operands)
+    import re
+    agg = {}
+    for n, t, by in zip(names, per_launch, lbytes):
+        key = re.sub(r"\[p\d\]|(?<=\))(A|B|AB)$", "", n)
+        a = agg.setdefault(key, [0.0, 0, 0])
+        a[0] += t
+        a[1] += by
+        a[2] += 1
+    dom_name = max(agg, key=lambda k: agg[k][0])
+    dom_ms, dom_bytes, dom_n = agg[dom_name]
+    dom_gbps = dom_bytes / (dom_ms * 1e-3) / 1e9
     workload = "plain_%dx%d_u%d" % (L, L, bits)
     traffic = load_traffic(workload)
     mbytes = model_bytes(plan, bits)
@@ -208,9 +236,11 @@ def main():
                    "N": plan.N, "primes": plan.primes, "passes": plan.passes},
         "latency_ms": {"p50": p50, "p99": p99},
         "bit_exact_vs_gmp": exact,
-        "kernels": [{"name": n, "avg_ms": t, "alg_bytes": b, "alg_GBps": b / (t * 1e-3) / 1e9}
-                    for n, t, b in zip(names, per_launch, lbytes)],
-        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": dom_gbps, "peak": HBM_PEAK_GBPS,
+        "kernels": [{"name": k, "launches": v[2], "ms_per_product": v[0], "alg_bytes": v[1],
+                     "alg_GBps": v[1] / (v[0] * 1e-3) / 1e9} for k, v in agg.items()],
+        "roofline": {"bound": "hbm", "kernel": dom_name, "launches_per_product": dom_n,
+                     "avg_launch_ms": dom_ms / dom_n, "alg_bytes_per_launch": dom_bytes / dom_n,
+                     "achieved": dom_gbps, "peak": HBM_PEAK_GBPS,
                      "unit": "GB/s", "frac": dom_gbps / HBM_PEAK_GBPS,
                      "traffic": traffic.get("hbm_bytes_per_launch") if traffic else None},
         "product_roofline": {"alg_bytes": mbytes, "achieved": mbytes / (ms_step * 1e-3) / 1e9,

@@ -71,6 +71,7 @@ SIGNATURES = {
     "bw_execute_crt": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "bw_execute_host": (_i, [_vp, _vp, _vp, _vp]),
     "bw_plan_status": (_i, [_vp]),
+    "bw_plan_set_schedule": (_i, [_vp, _i]),
     "bw_plan_set_timing": (_i, [_vp, _i]),
     "bw_plan_timing": (_i, [_vp, C.POINTER(C.c_float), _i, C.POINTER(_i)]),
     "bw_multiply_host": (_i, [_i, _vp, _u64, _vp, _u64, _vp, C.POINTER(C.c_double)]),
@@ -235,6 +236,10 @@ class Plan:
         _check(lib().bw_spectrum_order(self._p, order.ctypes.data_as(C.POINTER(C.c_uint64))))
         return order
 
+    def set_schedule(self, schedule):
+        """0: one launch per pass over all primes; 1: per-prime chains (default)."""
+        _check(lib().bw_plan_set_schedule(self._p, schedule))
+
     def status(self):
         _check(lib().bw_plan_status(self._p))
 
@@ -242,9 +247,9 @@ class Plan:
         _check(lib().bw_plan_set_timing(self._p, 1 if on else 0))
 
     def timing(self):
-        ms = (C.c_float * 16)()
+        ms = (C.c_float * 64)()
         n = C.c_int()
-        _check(lib().bw_plan_timing(self._p, ms, 16, C.byref(n)))
+        _check(lib().bw_plan_timing(self._p, ms, 64, C.byref(n)))
         return [float(ms[i]) for i in range(n.value)]
 
 

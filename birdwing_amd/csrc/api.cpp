@@ -167,7 +167,7 @@ extern "C" int bw_multiply_size_supported(int set, int in_bits, uint64_t L_A, ui
 
 // ---------------------------------------------------------------------------
 // plans
-#define BW_MAX_LAUNCH 12
+#define BW_MAX_LAUNCH 32
 
 struct bw_plan {
     int limb_bits = 32;
@@ -191,6 +191,7 @@ struct bw_plan {
     void* h_b = nullptr;
     void* h_c = nullptr;
     hipStream_t own = nullptr;
+    int schedule = BW_SCHED_PRIME_MAJOR;
     // timing: one event set (launches + 1 events) per recorded execution
     bool timing = false;
     std::vector<std::vector<hipEvent_t>> evpool;
@@ -198,20 +199,13 @@ struct bw_plan {
     int nev = 0;               // events recorded by the current execution
 };
 
+// Default pass split: passes of at most 2^9 points (register-staged kernels,
+// column tiles of 16+ adjacent columns), balanced, the largest share last.
 static void default_passes(int logN, std::vector<int>& out) {
     out.clear();
-    if (logN <= BW_MAX_PASS_LOG) { out.push_back(logN); return; }
-    if (logN <= 2 * BW_MAX_PASS_LOG) {
-        const int t2 = logN - 1 < BW_MAX_PASS_LOG ? logN - 1 : BW_MAX_PASS_LOG;
-        out.push_back(logN - t2);
-        out.push_back(t2);
-        return;
-    }
-    const int r = logN - BW_MAX_PASS_LOG;
-    const int t2 = r - 1 < BW_MAX_PASS_LOG ? r - 1 : BW_MAX_PASS_LOG;
-    out.push_back(r - t2);
-    out.push_back(t2);
-    out.push_back(BW_MAX_PASS_LOG);
+    const int P = (logN + 8) / 9 < 1 ? 1 : (logN + 8) / 9;
+    const int base = logN / P, extra = logN % P;
+    for (int j = 0; j < P; j++) out.push_back(base + (j >= P - extra ? 1 : 0));
 }
 
 static void plan_free(bw_plan* p) {
@@ -319,7 +313,9 @@ extern "C" void bw_plan_destroy(bw_plan* p) { plan_free(p); }
 
 static int count_launches(const bw_plan* p) {
     const int P = (int)p->passes.size();
-    return P == 1 ? 2 : (P - 1) + 1 + (P - 1) + 1;
+    if (P == 1) return 2;
+    if (p->schedule == BW_SCHED_PRIME_MAJOR) return p->set->k * (2 * (P - 1) + 1 + (P - 1)) + 1;
+    return (P - 1) + 1 + (P - 1) + 1;
 }
 
 extern "C" int bw_plan_get_info(const bw_plan* p, bw_plan_info* info) {
@@ -358,9 +354,14 @@ static PassPlan pass_geom(const bw_plan* p, int j) {
     g.logM = p->logN - prefix;
     g.S = (int64_t)1 << (p->logN - prefix - g.logT);
     g.O = (int64_t)1 << prefix;
-    const int gmax = bw_pass_max_G(g.logT);
-    if (g.S > 1) g.G = (int)(g.S < gmax ? g.S : gmax);
-    else g.G = (int)(g.O < gmax ? g.O : gmax);
+    int gmax = bw_pass_max_G(g.logT);
+    if (g.S > 1) {
+        g.G = (int)(g.S < gmax ? g.S : gmax);
+    } else {
+        // rows: G * T <= 4096 keeps the fused kernel's per-thread registers small
+        while (gmax > 1 && ((int64_t)gmax << g.logT) > 4096) gmax >>= 1;
+        g.G = (int)(g.O < gmax ? g.O : gmax);
+    }
     g.tiles = g.O * g.S / g.G;
     return g;
 }
@@ -427,6 +428,12 @@ struct Marker {
     } while (0)
 
 // Transform + product + inverse, leaving canonical residues in bufA.
+//
+// BW_SCHED_PASS_MAJOR: one launch per pass over every (prime, operand).
+// BW_SCHED_PRIME_MAJOR: the chain of one (prime, operand) -- forward passes of
+// A_p, of B_p, the fused pass, the inverse passes of prime p -- runs before the
+// next prime starts, so each 128 MiB intermediate (at N = 2^25) is re-read
+// while it is still held by the 256 MB Infinity Cache.
 static int run_transforms(bw_plan* p, const void* d_a, const void* d_b, hipStream_t st, Marker& mk) {
     const int P = (int)p->passes.size();
     if (P == 1) {
@@ -442,41 +449,62 @@ static int run_transforms(bw_plan* p, const void* d_a, const void* d_b, hipStrea
         LAUNCH_TRY(mk.mark());
         return BW_OK;
     }
-    for (int j = 0; j < P - 1; j++) {
-        PassPlan g = pass_geom(p, j);
-        PassParams prm = base_params(p, g);
-        prm.nops = 2;
-        if (j == 0) {
-            prm.ingress = 1;
-            prm.in[0] = d_a;
-            prm.in[1] = d_b;
-            prm.in_len[0] = (int64_t)p->L_A;
-            prm.in_len[1] = (int64_t)p->L_B;
-        } else {
+    const bool folded = pass_geom(p, P - 1).logT <= BW_REG2_MAX_LOG && pass_geom(p, P - 2).logT <= BW_REG2_MAX_LOG;
+    const bool pm = p->schedule == BW_SCHED_PRIME_MAJOR;
+    const int k = p->set->k;
+    const int nchunk = pm ? k : 1;          // prime chunks
+    const int nper = pm ? 1 : k;            // primes per launch
+    const void* raw[2] = {d_a, d_b};
+    void* buf[2] = {p->bufA, p->bufB};
+    const int64_t len[2] = {(int64_t)p->L_A, (int64_t)p->L_B};
+    for (int c = 0; c < nchunk; c++) {
+        const int prime0 = pm ? c : 0;
+        // forward passes 1 .. P-1: per operand (prime-major) or both operands at once
+        for (int op = 0; op < (pm ? 2 : 1); op++) {
+            for (int j = 0; j < P - 1; j++) {
+                PassPlan g = pass_geom(p, j);
+                PassParams prm = base_params(p, g);
+                prm.prime0 = prime0;
+                prm.canon_out = 0;
+                const int nops = pm ? 1 : 2;
+                for (int q = 0; q < nops; q++) {
+                    const int which = pm ? op : q;
+                    prm.in[q] = j == 0 ? raw[which] : (const void*)buf[which];
+                    prm.in_len[q] = len[which];
+                    prm.out[q] = buf[which];
+                }
+                prm.ingress = j == 0;
+                prm.nops = nops;
+                PassLaunch L = {BW_MODE_FWD, g.logT, g.G, p->limb_bits, nper, nops, g.tiles};
+                LAUNCH_TRY(bw_launch_pass(L, prm, st));
+                LAUNCH_TRY(mk.mark());
+            }
+        }
+        {
+            PassPlan g = pass_geom(p, P - 1);
+            PassParams prm = base_params(p, g);
+            prm.prime0 = prime0;
             prm.in[0] = p->bufA;
             prm.in[1] = p->bufB;
+            prm.out[0] = p->bufA;
+            prm.canon_out = 0;
+            prm.fold_ninv = folded;
+            PassLaunch L = {BW_MODE_FUSED, g.logT, g.G, p->limb_bits, nper, 1, g.tiles};
+            LAUNCH_TRY(bw_launch_pass(L, prm, st));
+            LAUNCH_TRY(mk.mark());
         }
-        prm.out[0] = p->bufA;
-        prm.out[1] = p->bufB;
-        LAUNCH_TRY(launch(p, BW_MODE_FWD, g, prm, 2, st));
-        LAUNCH_TRY(mk.mark());
-    }
-    {
-        PassPlan g = pass_geom(p, P - 1);
-        PassParams prm = base_params(p, g);
-        prm.in[0] = p->bufA;
-        prm.in[1] = p->bufB;
-        prm.out[0] = p->bufA;
-        LAUNCH_TRY(launch(p, BW_MODE_FUSED, g, prm, 1, st));
-        LAUNCH_TRY(mk.mark());
-    }
-    for (int j = P - 2; j >= 0; j--) {
-        PassPlan g = pass_geom(p, j);
-        PassParams prm = base_params(p, g);
-        prm.in[0] = p->bufA;
-        prm.out[0] = p->bufA;
-        LAUNCH_TRY(launch(p, BW_MODE_INV, g, prm, 1, st));
-        LAUNCH_TRY(mk.mark());
+        for (int j = P - 2; j >= 0; j--) {
+            PassPlan g = pass_geom(p, j);
+            PassParams prm = base_params(p, g);
+            prm.prime0 = prime0;
+            prm.in[0] = p->bufA;
+            prm.out[0] = p->bufA;
+            prm.canon_out = j == 0 ? 1 : 0;
+            prm.fold_ninv = (j == P - 2) && folded;
+            PassLaunch L = {BW_MODE_INV, g.logT, g.G, p->limb_bits, nper, 1, g.tiles};
+            LAUNCH_TRY(bw_launch_pass(L, prm, st));
+            LAUNCH_TRY(mk.mark());
+        }
     }
     return BW_OK;
 }
@@ -554,6 +582,15 @@ extern "C" int bw_execute_host(bw_plan* p, const void* a, const void* b, void* c
     if ((rc = bw_execute(p, p->h_a, p->h_b, p->h_c, p->own))) return rc;
     HIP_TRY(hipMemcpyAsync(c, p->h_c, (p->L_A + p->L_B) * wb, hipMemcpyDeviceToHost, p->own));
     HIP_TRY(hipStreamSynchronize(p->own));
+    return BW_OK;
+}
+
+extern "C" int bw_plan_set_schedule(bw_plan* p, int schedule) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (schedule != BW_SCHED_PASS_MAJOR && schedule != BW_SCHED_PRIME_MAJOR)
+        return fail(BW_ERR_ARG, "unknown schedule %d", schedule);
+    p->schedule = schedule;
     return BW_OK;
 }
 
@@ -666,7 +703,7 @@ extern "C" int bw_spectrum_order(const bw_plan* p, uint64_t* order) {
         uint64_t f = 0, mult = 1;
         for (int j = 0; j < P; j++) {
             const int64_t qj = (int64_t)(q / S[j]) & (((int64_t)1 << p->passes[j]) - 1);
-            f += (uint64_t)sched_freq_of_pos(p->passes[j], qj) * mult;
+            f += (uint64_t)bw_pass_freq_of_pos(p->passes[j], qj) * mult;
             mult <<= p->passes[j];
         }
         order[q] = f;

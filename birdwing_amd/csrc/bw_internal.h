@@ -10,18 +10,18 @@
 #define BW_TW_LO_LOG 12          // pass twiddles: 2-level table split at 2^12
 #define BW_MAX_PASS_LOG 13       // largest sub-transform held in one workgroup
 
-// Per-prime constants of the FP64 (u32 digit) path.
+// Per-prime constants of the FP64 (u32 digit) path: roots of order up to 32.
 struct F64PrimeDev {
     F64Prime P;
-    double w16[8][2];    // omega_16^e, e < 8: (w, w/p)
-    double iw16[8][2];   // omega_16^-e
+    double w32[16][2];    // omega_32^e, e < 16: (w, w/p)
+    double iw32[16][2];   // omega_32^-e
 };
 
 // Per-prime constants of the u64 digit path (Shoup pairs (w, floor(w 2^64/p))).
 struct U64PrimeDev {
     U64Prime P;
-    uint64_t w16[8][2];
-    uint64_t iw16[8][2];
+    uint64_t w32[16][2];
+    uint64_t iw32[16][2];
 };
 
 // Device tables for one prime set, shared by every plan that uses the set.
@@ -66,7 +66,8 @@ struct PassParams {
     void* out[2];
     int64_t in_len[2];     // raw limb counts when ingress != 0
     int ingress;           // 1: inputs are raw limbs (zero-padded to N, pass 1 only)
-    int canon_out;         // 1: store canonical residues in [0, p)
+    int canon_out;         // 1: store canonical residues in [0, p); 0: centered (32-bit path)
+    int fold_ninv;         // FUSED: skip N^-1 (the next INV pass applies it); INV: apply N^-1
     int nops;              // FWD: operands in this launch
     int64_t N;             // transform length
     int64_t plane;         // elements per prime plane (N, or batch stride)

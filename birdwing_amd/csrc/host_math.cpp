@@ -193,16 +193,16 @@ void build_tables(const PrimeSet& s, HostTables& out) {
         const uint64_t wstep = powmod(wL, nlo, p);
         x = 1;
         for (size_t e = 0; e < nhi; e++) { put(out.tw_hi, i * nhi + e, x, p); x = mulmod(x, wstep, p); }
-        const uint64_t w16 = powmod(wL, 1ULL << (L - 4), p), iw16 = invmod(w16, p);
+        const uint64_t w32 = powmod(wL, 1ULL << (L - 5), p), iw32 = invmod(w32, p);
         if (f64) {
             F64PrimeDev c;
             std::memset(&c, 0, sizeof(c));
             f64_prime(p, c.P);
             uint64_t a = 1, b = 1;
-            for (int e = 0; e < 8; e++) {
-                c.w16[e][0] = (double)a; c.w16[e][1] = (double)a / (double)p;
-                c.iw16[e][0] = (double)b; c.iw16[e][1] = (double)b / (double)p;
-                a = mulmod(a, w16, p); b = mulmod(b, iw16, p);
+            for (int e = 0; e < 16; e++) {
+                c.w32[e][0] = (double)a; c.w32[e][1] = (double)a / (double)p;
+                c.iw32[e][0] = (double)b; c.iw32[e][1] = (double)b / (double)p;
+                a = mulmod(a, w32, p); b = mulmod(b, iw32, p);
             }
             std::memcpy(out.consts.data() + i * sizeof(F64PrimeDev), &c, sizeof(c));
         } else {
@@ -215,10 +215,10 @@ void build_tables(const PrimeSet& s, HostTables& out) {
             c.P.r2 = (uint64_t)((((u128)1 << 64) % p) * (u128)(((u128)1 << 64) % p) % p);
             c.P.floor_r = (uint64_t)(~0ULL / p);
             uint64_t a = 1, b = 1;
-            for (int e = 0; e < 8; e++) {
-                c.w16[e][0] = a; c.w16[e][1] = (uint64_t)(((u128)a << 64) / p);
-                c.iw16[e][0] = b; c.iw16[e][1] = (uint64_t)(((u128)b << 64) / p);
-                a = mulmod(a, w16, p); b = mulmod(b, iw16, p);
+            for (int e = 0; e < 16; e++) {
+                c.w32[e][0] = a; c.w32[e][1] = (uint64_t)(((u128)a << 64) / p);
+                c.iw32[e][0] = b; c.iw32[e][1] = (uint64_t)(((u128)b << 64) / p);
+                a = mulmod(a, w32, p); b = mulmod(b, iw32, p);
             }
             std::memcpy(out.consts.data() + i * sizeof(U64PrimeDev), &c, sizeof(c));
         }

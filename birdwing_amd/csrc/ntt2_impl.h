@@ -1,0 +1,458 @@
+// ntt2_impl.h -- register-staged pass kernels for sub-transforms of T <= 1024.
+//
+// Same pass contract as ntt_pass_impl.h (element (o, t, s) at (o*T + t)*S + s,
+// forward output at digit-reversed positions, inverse consuming them), built
+// for throughput:
+//
+//   T = R0 * R1 (R0, R1 <= 32).  Stage 0 (radix R0, stride R1) reads its group
+//   straight from HBM into registers; one padded LDS transpose hands stage 1
+//   (radix R1, stride 1) its groups; stage 1 writes straight to HBM.  A tile
+//   is G lines (G adjacent columns when S > 1, so every row segment is G*4
+//   contiguous bytes; G adjacent rows when S == 1).
+//
+//   Position q = k0*R1 + k1 of a line holds frequency f = k0 + R0*k1.  The pass
+//   twiddle w_M^{f s} for the R1 outputs of one stage-1 group (fixed k0, s) is
+//   base * step^k1 with base = w_M^{k0 s} and step = w_M^{R0 s}: two table
+//   look-ups and one modular product per element, no gathers.
+#pragma once
+#include <type_traits>
+#include "ntt_pass_impl.h"
+
+// Arithmetic policies keyed on the prime constants themselves.
+struct ArF64x {
+    using V = double;          // register value: signed integer held in FP64
+    using W = uint32_t;        // storage word (canonical residue)
+    using Raw = uint32_t;      // raw input limb (< 2^32: enters unreduced)
+    using TW = double2;        // twiddle (w, w/p)
+    using NI = double;
+    using PC32 = F64PrimeDev;
+    using PR = F64Prime;
+    using LV = int32_t;        // LDS element: centered residue (halves LDS per wavefront)
+    __device__ static LV to_lds(V x, const PR& P) { return (int32_t)f64_reduce(x, P); }
+    __device__ static V from_lds(LV x) { return (double)x; }
+    // words hold canonical [0, p) or centered residues; both read as int32
+    __device__ static V from_word(W x, const PR&) { return (double)(int32_t)x; }
+    __device__ static V from_raw(Raw x, const PR&) { return (double)x; }
+    __device__ static W to_word(V x, const PR& P) { return f64_canon(x, P); }
+    __device__ static W to_word_c(V x, const PR& P, bool canon) {
+        return canon ? f64_canon(x, P) : (uint32_t)(int32_t)f64_reduce(x, P);
+    }
+    // a*b (no N^-1) for a, b < 2^46 in magnitude
+    __device__ static V mul_plain(V a, V b, const PR& P) {
+        const double br = f64_reduce(b, P);
+        return f64_mulmod(a, br, br * P.pinv, P);
+    }
+    __device__ static TW tw_mul_ninv(TW t, const NI* ninv, const PR& P) {
+        const double w = f64_mulmod(t.x, ninv[0], ninv[1], P);
+        return make_double2(w, w * P.pinv);
+    }
+    __device__ static V add(V a, V b, const PR&) { return a + b; }
+    __device__ static V sub(V a, V b, const PR&) { return a - b; }
+    __device__ static V mul(V y, TW w, const PR& P) { return f64_mulmod(y, w.x, w.y, P); }
+    __device__ static V mulc(V y, const double* w, const PR& P) { return f64_mulmod(y, w[0], w[1], P); }
+    __device__ static V mul_scaled(V a, V b, const NI* ninv, const PR& P) {
+        const double br = f64_reduce(b, P);
+        const double r = f64_mulmod(a, br, br * P.pinv, P);
+        return f64_mulmod(r, ninv[0], ninv[1], P);
+    }
+    __device__ static V zero() { return 0.0; }
+    struct Gen {          // w_k = base * step^k, values in (-p, p)
+        double w, step, stepp;
+        __device__ void init(double2 base, double2 st, const PR&) { w = base.x; step = st.x; stepp = st.y; }
+        __device__ double apply(double x, const PR& P) const { return f64_mulmod(x, w, w * P.pinv, P); }
+        __device__ void next(const PR& P) { w = f64_mulmod(w, step, stepp, P); }
+    };
+    __device__ static TW twprod(TW a, TW b, const PR& P) {
+        const double w = f64_mulmod(a.x, b.x, b.y, P);
+        return make_double2(w, w * P.pinv);
+    }
+    // twiddles base * step^k, k < n (n a multiple of 4), as four interleaved
+    // chains of stride step^4 (short dependency chains)
+    template <int NTW>
+    __device__ static void gen_powers(TW base, TW step, double* w, const PR& P) {
+        const double s = step.x, sp = step.y;
+        const double s2 = f64_mulmod(s, s, sp, P), s2p = s2 * P.pinv;
+        const double s4 = f64_mulmod(s2, s2, s2p, P), s4p = s4 * P.pinv;
+        w[0] = base.x;
+        w[1] = f64_mulmod(base.x, s, sp, P);
+        w[2] = f64_mulmod(base.x, s2, s2p, P);
+        w[3] = f64_mulmod(w[1], s2, s2p, P);
+#pragma unroll
+        for (int k = 4; k < NTW; k++) w[k] = f64_mulmod(w[k - 4], s4, s4p, P);
+    }
+    __device__ static V apply_tw(V x, double w, const PR& P) { return f64_mulmod(x, w, w * P.pinv, P); }
+    __device__ static const PR& prime(const PC32& c) { return c.P; }
+};
+
+struct ArU64x {
+    using V = uint64_t;
+    using W = uint64_t;
+    using Raw = uint64_t;
+    using TW = ulonglong2;     // Shoup pair (w, floor(w 2^64/p))
+    using NI = uint64_t;
+    using PC32 = U64PrimeDev;
+    using PR = U64Prime;
+    using LV = uint64_t;
+    __device__ static LV to_lds(V x, const PR&) { return x; }
+    __device__ static V from_lds(LV x) { return x; }
+    __device__ static V from_word(W x, const PR&) { return x; }
+    __device__ static V from_raw(Raw x, const PR& P) { return u64_reduce(x, P); }
+    __device__ static W to_word(V x, const PR&) { return x; }
+    __device__ static W to_word_c(V x, const PR&, bool) { return x; }
+    // montmul(montmul(a, b), 2^128) = a*b
+    __device__ static V mul_plain(V a, V b, const PR& P) { return u64_montmul(u64_montmul(a, b, P), P.r2, P); }
+    // ninv[0] = N^-1 2^128 mod p: montmul(t, ninv) = t N^-1 2^64 -> montmul by 1 -> t N^-1
+    __device__ static TW tw_mul_ninv(TW t, const NI* ninv, const PR& P) {
+        ulonglong2 r;
+        r.x = u64_montmul(u64_montmul(t.x, ninv[0], P), 1, P);
+        r.y = 0;
+        return r;
+    }
+    __device__ static V add(V a, V b, const PR& P) { return u64_add(a, b, P.p); }
+    __device__ static V sub(V a, V b, const PR& P) { return u64_sub(a, b, P.p); }
+    __device__ static V mul(V y, TW w, const PR& P) { return u64_mulmod_shoup(y, w.x, w.y, P.p); }
+    __device__ static V mulc(V y, const uint64_t* w, const PR& P) { return u64_mulmod_shoup(y, w[0], w[1], P.p); }
+    __device__ static V mul_scaled(V a, V b, const NI* ninv, const PR& P) {
+        return u64_montmul(u64_montmul(a, b, P), ninv[0], P);
+    }
+    __device__ static V zero() { return 0; }
+    struct Gen {          // Montgomery form: wm = w 2^64 mod p
+        uint64_t wm, stepm;
+        __device__ void init(ulonglong2 base, ulonglong2 st, const PR& P) {
+            wm = u64_montmul(base.x, P.r2, P);
+            stepm = u64_montmul(st.x, P.r2, P);
+        }
+        __device__ uint64_t apply(uint64_t x, const PR& P) const { return u64_montmul(x, wm, P); }
+        __device__ void next(const PR& P) { wm = u64_montmul(wm, stepm, P); }
+    };
+    __device__ static TW twprod(TW a, TW b, const PR& P) {
+        ulonglong2 r;
+        r.x = u64_mulmod_shoup(a.x, b.x, b.y, P.p);
+        r.y = 0;   // only .x is consumed
+        return r;
+    }
+    // Montgomery-form twiddles base * step^k (four interleaved chains)
+    template <int NTW>
+    __device__ static void gen_powers(TW base, TW step, uint64_t* w, const PR& P) {
+        const uint64_t s = u64_montmul(step.x, P.r2, P);
+        const uint64_t s2 = u64_montmul(s, s, P), s4 = u64_montmul(s2, s2, P);
+        w[0] = u64_montmul(base.x, P.r2, P);
+        w[1] = u64_montmul(w[0], s, P);
+        w[2] = u64_montmul(w[0], s2, P);
+        w[3] = u64_montmul(w[1], s2, P);
+#pragma unroll
+        for (int k = 4; k < NTW; k++) w[k] = u64_montmul(w[k - 4], s4, P);
+    }
+    __device__ static V apply_tw(V x, uint64_t w, const PR& P) { return u64_montmul(x, w, P); }
+    __device__ static const PR& prime(const PC32& c) { return c.P; }
+};
+
+// Codelets with roots up to order 32 (R <= 32).
+template <class AR, int RLOG>
+__device__ __forceinline__ void dft32_dif(typename AR::V* x, const typename AR::PC32& pc) {
+    constexpr int R = 1 << RLOG;
+#pragma unroll
+    for (int q = 0; q < RLOG; q++) {
+        const int h = R >> (q + 1);
+#pragma unroll
+        for (int blk = 0; blk < R; blk += 2 * h) {
+#pragma unroll
+            for (int i = 0; i < h; i++) {
+                auto a = x[blk + i], b = x[blk + i + h];
+                x[blk + i] = AR::add(a, b, pc.P);
+                auto d = AR::sub(a, b, pc.P);
+                x[blk + i + h] = (i == 0) ? d : AR::mulc(d, pc.w32[i * (16 / h)], pc.P);
+            }
+        }
+    }
+}
+
+template <class AR, int RLOG>
+__device__ __forceinline__ void dft32_dit_inv(typename AR::V* x, const typename AR::PC32& pc) {
+    constexpr int R = 1 << RLOG;
+#pragma unroll
+    for (int q = RLOG - 1; q >= 0; q--) {
+        const int h = R >> (q + 1);
+#pragma unroll
+        for (int blk = 0; blk < R; blk += 2 * h) {
+#pragma unroll
+            for (int i = 0; i < h; i++) {
+                auto a = x[blk + i], b = x[blk + i + h];
+                if (i != 0) b = AR::mulc(b, pc.iw32[i * (16 / h)], pc.P);
+                x[blk + i] = AR::add(a, b, pc.P);
+                x[blk + i + h] = AR::sub(a, b, pc.P);
+            }
+        }
+    }
+}
+
+
+// LDS line: pad one slot per R1 elements, odd line stride.
+template <int LOGT>
+struct T2Lds {
+    static constexpr int R1 = 1 << t2_r1log(LOGT);
+    static constexpr int LT = (1 << LOGT) + (1 << LOGT) / R1 + 1;
+    __device__ static int pos(int q) { return q + q / R1; }
+};
+
+// w_M^e from the two-level table, e < 2^table_log.
+template <class AR>
+__device__ __forceinline__ typename AR::TW tw_lookup(int64_t e, const typename AR::TW* lo, const typename AR::TW* hi,
+                                                     const typename AR::PC32& pc) {
+    const typename AR::TW a = lo[e & ((1 << BW_TW_LO_LOG) - 1)];
+    const int64_t eh = e >> BW_TW_LO_LOG;
+    return eh ? AR::twprod(a, hi[eh], AR::prime(pc)) : a;
+}
+
+// One workgroup = one tile of G lines (G a power of two, runtime) with
+// NT = G * R1 threads: thread i owns stage-0 group i and stage-1 groups
+// i, i + NT, ...  COLS (S > 1) and INGRESS (raw limbs in) are compile-time so
+// the inner loops carry no branches.
+template <class AR, int LOGT, int MODE, bool COLS, bool INGRESS>
+__global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
+    using V = typename AR::V;
+    using W = typename AR::W;
+    using TW = typename AR::TW;
+    constexpr int T = 1 << LOGT;
+    constexpr int R1LOG = t2_r1log(LOGT), R0LOG = t2_r0log(LOGT);
+    constexpr int R1 = 1 << R1LOG, R0 = 1 << R0LOG;
+    constexpr int LT = T2Lds<LOGT>::LT;
+    constexpr int IT1 = R0 > R1 ? R0 / R1 : 1;     // stage-1 groups per thread (upper bound)
+    constexpr int TWSH = BW_STAGE_LOG - LOGT;      // w_T = w_8192^(2^TWSH)
+    constexpr int TWMASK = (1 << BW_STAGE_LOG) - 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    typename AR::LV* lds = reinterpret_cast<typename AR::LV*>(smem_raw);
+
+    const int NT = blockDim.x;
+    const int logG = __builtin_ctz(NT) - R1LOG;
+    const int G = 1 << logG;
+    const int prime = prm.prime0 + blockIdx.y;
+    const int op = (MODE == BW_MODE_FWD) ? blockIdx.z : 0;
+    const typename AR::PC32 pc = reinterpret_cast<const typename AR::PC32*>(prm.consts)[prime];
+    const auto& P = AR::prime(pc);
+    const TW* tws = reinterpret_cast<const TW*>(prm.tw_stage) + ((int64_t)prime << BW_STAGE_LOG);
+    const TW* twlo = reinterpret_cast<const TW*>(prm.tw_lo) + ((int64_t)prime << BW_TW_LO_LOG);
+    const TW* twhi = reinterpret_cast<const TW*>(prm.tw_hi) + ((int64_t)prime << (prm.table_log - BW_TW_LO_LOG));
+    const int64_t S = prm.S;
+    const int64_t l0 = (int64_t)blockIdx.x << logG;
+    const int64_t o0 = COLS ? l0 / S : l0;
+    const int64_t s0 = COLS ? l0 - o0 * S : 0;
+    const int twshift = prm.table_log - prm.logM;
+    const int64_t plane_off = (int64_t)prime * prm.plane;
+    const int tid = threadIdx.x;
+    const bool canon = prm.canon_out != 0;
+    const typename AR::NI* ninv = reinterpret_cast<const typename AR::NI*>(prm.ninv) + 2 * prime;
+
+    // element t of line g lives at line_base(g) + t * tstride
+    const int64_t tstride = COLS ? S : 1;
+    auto line_base = [&](int g) -> int64_t { return COLS ? o0 * T * S + s0 + g : (o0 + g) * (int64_t)T; };
+    // stage-0 group of this thread: line g0, offset j
+    const int g0 = COLS ? (tid & (G - 1)) : (tid >> R1LOG);
+    const int j0 = COLS ? (tid >> logG) : (tid & (R1 - 1));
+
+    // stage-0 input words of one operand (raw limbs or residue words)
+    using IW = typename std::conditional<INGRESS, typename AR::Raw, W>::type;
+    auto load0 = [&](int which, IW* w) {
+        const int64_t base = line_base(g0) + j0 * tstride;
+        const int64_t step = (int64_t)R1 * tstride;
+        if constexpr (INGRESS) {
+            const typename AR::Raw* raw = reinterpret_cast<const typename AR::Raw*>(prm.in[which]);
+            const int64_t len = prm.in_len[which];
+#pragma unroll
+            for (int m = 0; m < R0; m++) {
+                const int64_t a = base + m * step;
+                w[m] = a < len ? raw[a] : (IW)0;
+            }
+        } else {
+            const W* in = reinterpret_cast<const W*>(prm.in[which]) + plane_off;
+#pragma unroll
+            for (int m = 0; m < R0; m++) w[m] = in[base + m * step];
+        }
+    };
+    auto fwd_stage0_from = [&](const IW* w) {
+        V x[R0];
+#pragma unroll
+        for (int m = 0; m < R0; m++) {
+            if constexpr (INGRESS) x[m] = AR::from_raw(w[m], P);
+            else x[m] = AR::from_word(w[m], P);
+        }
+        dft32_dif<AR, R0LOG>(x, pc);
+#pragma unroll
+        for (int k = 1; k < R0; k++) x[brev(k, R0LOG)] = AR::mul(x[brev(k, R0LOG)], tws[(j0 * k) << TWSH], P);
+#pragma unroll
+        for (int k = 0; k < R0; k++) lds[g0 * LT + T2Lds<LOGT>::pos(k * R1 + j0)] = AR::to_lds(x[brev(k, R0LOG)], P);
+    };
+    auto fwd_stage0 = [&](int which) {
+        const int64_t base = line_base(g0) + j0 * tstride;
+        const int64_t step = (int64_t)R1 * tstride;
+        V x[R0];
+        if constexpr (INGRESS) {
+            const typename AR::Raw* raw = reinterpret_cast<const typename AR::Raw*>(prm.in[which]);
+            const int64_t len = prm.in_len[which];
+#pragma unroll
+            for (int m = 0; m < R0; m++) {
+                const int64_t a = base + m * step;
+                x[m] = a < len ? AR::from_raw(raw[a], P) : AR::zero();
+            }
+        } else {
+            const W* in = reinterpret_cast<const W*>(prm.in[which]) + plane_off;
+#pragma unroll
+            for (int m = 0; m < R0; m++) x[m] = AR::from_word(in[base + m * step], P);
+        }
+        dft32_dif<AR, R0LOG>(x, pc);
+#pragma unroll
+        for (int k = 1; k < R0; k++) x[brev(k, R0LOG)] = AR::mul(x[brev(k, R0LOG)], tws[(j0 * k) << TWSH], P);
+#pragma unroll
+        for (int k = 0; k < R0; k++) lds[g0 * LT + T2Lds<LOGT>::pos(k * R1 + j0)] = AR::to_lds(x[brev(k, R0LOG)], P);
+    };
+    auto dec1 = [&](int gi, int& g, int& b) {
+        if (COLS) { g = gi & (G - 1); b = gi >> logG; } else { g = gi >> R0LOG; b = gi & (R0 - 1); }
+    };
+    // stage 1 forward: LDS -> DFT_R1 -> keep (fused) or pass twiddle -> HBM
+    auto fwd_stage1 = [&](int which, V* keep) {
+#pragma unroll
+        for (int it = 0; it < IT1; it++) {
+            const int gi = tid + it * NT;
+            if (gi >= G * R0) break;
+            int g, b;
+            dec1(gi, g, b);
+            V x[R1];
+#pragma unroll
+            for (int m = 0; m < R1; m++) x[m] = AR::from_lds(lds[g * LT + T2Lds<LOGT>::pos(b * R1 + m)]);
+            dft32_dif<AR, R1LOG>(x, pc);
+            if (keep) {
+#pragma unroll
+                for (int k = 0; k < R1; k++) keep[it * R1 + k] = x[brev(k, R1LOG)];
+                continue;
+            }
+            W* out = reinterpret_cast<W*>(prm.out[which]) + plane_off + line_base(g) + (int64_t)b * R1 * tstride;
+            if constexpr (COLS) {
+                const int64_t s = s0 + g;
+                typename AR::V tw[R1];
+                AR::template gen_powers<R1>(tw_lookup<AR>(((int64_t)b * s) << twshift, twlo, twhi, pc),
+                                            tw_lookup<AR>(((int64_t)R0 * s) << twshift, twlo, twhi, pc), tw, P);
+#pragma unroll
+                for (int k = 0; k < R1; k++) out[k * tstride] = AR::to_word_c(AR::apply_tw(x[brev(k, R1LOG)], tw[k], P), P, canon);
+            } else {
+#pragma unroll
+                for (int k = 0; k < R1; k++) out[k] = AR::to_word_c(x[brev(k, R1LOG)], P, canon);
+            }
+        }
+    };
+    // stage 1 inverse: keep (fused) or HBM with the inverse pass twiddle -> inverse DFT_R1 -> LDS
+    auto inv_stage1 = [&](const V* keep) {
+#pragma unroll
+        for (int it = 0; it < IT1; it++) {
+            const int gi = tid + it * NT;
+            if (gi >= G * R0) break;
+            int g, b;
+            dec1(gi, g, b);
+            V x[R1];
+            if (keep) {
+#pragma unroll
+                for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = keep[it * R1 + k];
+            } else {
+                const W* in = reinterpret_cast<const W*>(prm.in[0]) + plane_off + line_base(g) + (int64_t)b * R1 * tstride;
+                if constexpr (COLS) {
+                    const int64_t s = s0 + g;
+                    const int64_t mask = ((int64_t)1 << prm.table_log) - 1;
+                    TW base = tw_lookup<AR>((-(((int64_t)b * s) << twshift)) & mask, twlo, twhi, pc);
+                    if (prm.fold_ninv) base = AR::tw_mul_ninv(base, ninv, P);
+                    typename AR::V tw[R1];
+                    AR::template gen_powers<R1>(base, tw_lookup<AR>((-(((int64_t)R0 * s) << twshift)) & mask, twlo, twhi, pc),
+                                                tw, P);
+#pragma unroll
+                    for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::apply_tw(AR::from_word(in[k * tstride], P), tw[k], P);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::from_word(in[k], P);
+                }
+            }
+            dft32_dit_inv<AR, R1LOG>(x, pc);
+#pragma unroll
+            for (int m = 0; m < R1; m++) lds[g * LT + T2Lds<LOGT>::pos(b * R1 + m)] = AR::to_lds(x[m], P);
+        }
+    };
+    // stage 0 inverse: LDS -> inverse stage twiddle -> inverse DFT_R0 -> HBM
+    auto inv_stage0 = [&]() {
+        V x[R0];
+#pragma unroll
+        for (int k = 0; k < R0; k++) x[brev(k, R0LOG)] = AR::from_lds(lds[g0 * LT + T2Lds<LOGT>::pos(k * R1 + j0)]);
+#pragma unroll
+        for (int k = 1; k < R0; k++)
+            x[brev(k, R0LOG)] = AR::mul(x[brev(k, R0LOG)], tws[(-((j0 * k) << TWSH)) & TWMASK], P);
+        dft32_dit_inv<AR, R0LOG>(x, pc);
+        W* out = reinterpret_cast<W*>(prm.out[0]) + plane_off + line_base(g0) + j0 * tstride;
+        const int64_t step = (int64_t)R1 * tstride;
+#pragma unroll
+        for (int m = 0; m < R0; m++) out[m * step] = AR::to_word_c(x[m], P, canon);
+    };
+
+    if constexpr (MODE == BW_MODE_FWD) {
+        fwd_stage0(op);
+        __syncthreads();
+        fwd_stage1(op, nullptr);
+    } else if constexpr (MODE == BW_MODE_INV) {
+        inv_stage1(nullptr);
+        __syncthreads();
+        inv_stage0();
+    } else {
+        constexpr int KEEPN = IT1 * R1;
+        V keepA[KEEPN], keepB[KEEPN];
+#pragma unroll
+        for (int i = 0; i < KEEPN; i++) { keepA[i] = AR::zero(); keepB[i] = AR::zero(); }
+        IW wa[R0], wb[R0];
+        load0(0, wa);
+        load0(1, wb);                      // B's loads overlap A's transform
+        fwd_stage0_from(wa);
+        __syncthreads();
+        fwd_stage1(0, keepA);
+        __syncthreads();
+        fwd_stage0_from(wb);
+        __syncthreads();
+        fwd_stage1(1, keepB);
+        if (prm.fold_ninv) {
+#pragma unroll
+            for (int i = 0; i < KEEPN; i++) keepA[i] = AR::mul_plain(keepA[i], keepB[i], P);
+        } else {
+#pragma unroll
+            for (int i = 0; i < KEEPN; i++) keepA[i] = AR::mul_scaled(keepA[i], keepB[i], ninv, P);
+        }
+        __syncthreads();
+        inv_stage1(keepA);
+        __syncthreads();
+        inv_stage0();
+    }
+}
+
+typedef void (*PassKernel2)(PassParams);
+
+template <class AR, int MODE, int LOGT>
+static PassKernel2 pick2_v(bool cols, bool ingress) {
+    if constexpr (MODE == BW_MODE_INV) {
+        return cols ? ntt2_kernel<AR, LOGT, MODE, true, false> : ntt2_kernel<AR, LOGT, MODE, false, false>;
+    } else if constexpr (MODE == BW_MODE_FUSED) {
+        if (cols) return nullptr;
+        return ingress ? ntt2_kernel<AR, LOGT, MODE, false, true> : ntt2_kernel<AR, LOGT, MODE, false, false>;
+    } else {
+        if (cols) return ingress ? ntt2_kernel<AR, LOGT, MODE, true, true> : ntt2_kernel<AR, LOGT, MODE, true, false>;
+        return ingress ? ntt2_kernel<AR, LOGT, MODE, false, true> : ntt2_kernel<AR, LOGT, MODE, false, false>;
+    }
+}
+
+template <class AR, int MODE>
+static PassKernel2 pick2(int logT, bool cols, bool ingress) {
+    switch (logT) {
+    case 1: return pick2_v<AR, MODE, 1>(cols, ingress);
+    case 2: return pick2_v<AR, MODE, 2>(cols, ingress);
+    case 3: return pick2_v<AR, MODE, 3>(cols, ingress);
+    case 4: return pick2_v<AR, MODE, 4>(cols, ingress);
+    case 5: return pick2_v<AR, MODE, 5>(cols, ingress);
+    case 6: return pick2_v<AR, MODE, 6>(cols, ingress);
+    case 7: return pick2_v<AR, MODE, 7>(cols, ingress);
+    case 8: return pick2_v<AR, MODE, 8>(cols, ingress);
+    case 9: return pick2_v<AR, MODE, 9>(cols, ingress);
+    case 10: return pick2_v<AR, MODE, 10>(cols, ingress);
+    default: return nullptr;
+    }
+}

@@ -57,3 +57,21 @@ struct PassLaunch {
 hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_t stream);
 // Largest lines-per-workgroup the kernel set provides for a pass of 2^logT points.
 int bw_pass_max_G(int logT);
+// Passes of up to 2^BW_REG2_MAX_LOG points use the register-staged kernels
+// (ntt2_impl.h), larger ones the LDS radix-16 kernels (ntt_pass_impl.h).
+#define BW_REG2_MAX_LOG 10
+// Radix split T = R0 * R1 of the register-staged kernels: R1 (stage 1,
+// stride 1) takes up to 16 points below T = 2^9 and 32 points from 2^9 on, so
+// the stage-0 radix R0 (which sets the LDS bytes per wavefront) stays <= 32.
+__host__ __device__ constexpr int t2_r1log(int logT) { return logT <= 5 ? logT : (logT <= 8 ? 4 : 5); }
+__host__ __device__ constexpr int t2_r0log(int logT) { return logT - t2_r1log(logT); }
+int bw_reg2_lds_line(int logT);
+// Frequency held at position q of a pass line (either kernel family).
+__host__ __device__ inline int64_t bw_pass_freq_of_pos(int logT, int64_t q) {
+    if (logT <= BW_REG2_MAX_LOG) {
+        const int r1l = t2_r1log(logT);
+        const int r0l = logT - r1l;
+        return (q >> r1l) + ((q & ((1 << r1l) - 1)) << r0l);
+    }
+    return sched_freq_of_pos(logT, q);
+}

@@ -8,16 +8,42 @@ PassKernel bw_pick_f64_fused(int logT, int G);
 PassKernel bw_pick_u64_fwd(int logT, int G);
 PassKernel bw_pick_u64_inv(int logT, int G);
 PassKernel bw_pick_u64_fused(int logT, int G);
+PassKernel bw_pick2_f64_fwd(int logT, bool cols, bool ingress);
+PassKernel bw_pick2_f64_inv(int logT, bool cols, bool ingress);
+PassKernel bw_pick2_f64_fused(int logT, bool cols, bool ingress);
+PassKernel bw_pick2_u64_fwd(int logT, bool cols, bool ingress);
+PassKernel bw_pick2_u64_inv(int logT, bool cols, bool ingress);
+PassKernel bw_pick2_u64_fused(int logT, bool cols, bool ingress);
 
 int bw_pass_max_G(int logT) {
+    if (logT <= BW_REG2_MAX_LOG) {
+        // register-staged: G * T <= 8192 (LDS) and G * R1 <= 512 threads
+        const int r1 = t2_r1log(logT);
+        int g = 512 >> r1;
+        while (g > 1 && (1 << logT) * g > 8192) g >>= 1;
+        return g;
+    }
     int g = 16;
     while (g > 1 && (1 << logT) * g > 8192) g >>= 1;
     return g;
 }
 
+int bw_reg2_lds_line(int logT) { return (1 << logT) + ((1 << logT) >> t2_r1log(logT)) + 1; }
+
 hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_t stream) {
     PassKernel k = nullptr;
-    if (L.limb_bits == 64) {
+    const bool reg2 = L.logT <= BW_REG2_MAX_LOG;   // register-staged kernels
+    if (reg2) {
+        if (L.limb_bits == 64) {
+            if (L.mode == BW_MODE_FWD) k = bw_pick2_u64_fwd(L.logT, prm.S > 1, prm.ingress != 0);
+            else if (L.mode == BW_MODE_INV) k = bw_pick2_u64_inv(L.logT, prm.S > 1, prm.ingress != 0);
+            else k = bw_pick2_u64_fused(L.logT, prm.S > 1, prm.ingress != 0);
+        } else {
+            if (L.mode == BW_MODE_FWD) k = bw_pick2_f64_fwd(L.logT, prm.S > 1, prm.ingress != 0);
+            else if (L.mode == BW_MODE_INV) k = bw_pick2_f64_inv(L.logT, prm.S > 1, prm.ingress != 0);
+            else k = bw_pick2_f64_fused(L.logT, prm.S > 1, prm.ingress != 0);
+        }
+    } else if (L.limb_bits == 64) {
         if (L.mode == BW_MODE_FWD) k = bw_pick_u64_fwd(L.logT, L.G);
         else if (L.mode == BW_MODE_INV) k = bw_pick_u64_inv(L.logT, L.G);
         else k = bw_pick_u64_fused(L.logT, L.G);
@@ -27,10 +53,11 @@ hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_
         else k = bw_pick_f64_fused(L.logT, L.G);
     }
     if (!k) return hipErrorInvalidValue;
-    const size_t vbytes = 8;   // FP64 and u64 register values are both 8 bytes
-    const size_t lds = (size_t)L.G * lds_line(L.logT) * vbytes;
-    static bool attr_set[2][3][16][17] = {};
-    bool& done = attr_set[L.limb_bits == 64][L.mode][L.logT][L.G];
+    // LDS element: 8 bytes, except the register-staged 32-bit path (centered int32)
+    const size_t vbytes = (reg2 && L.limb_bits == 32) ? 4 : 8;
+    const size_t lds = (size_t)L.G * (reg2 ? bw_reg2_lds_line(L.logT) : lds_line(L.logT)) * vbytes;
+    static bool attr_set[2][3][16][33][4] = {};
+    bool& done = attr_set[L.limb_bits == 64][L.mode][L.logT][L.G][(prm.S > 1) * 2 + (prm.ingress != 0)];
     if (!done) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -38,6 +65,9 @@ hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_
         done = true;
     }
     dim3 grid((unsigned)L.tiles, (unsigned)L.nprimes, (unsigned)(L.mode == BW_MODE_FWD ? L.nops : 1));
-    hipLaunchKernelGGL(k, grid, dim3(256), lds, stream, prm);
+    // register-staged kernels run G * R1 threads (one stage-0 group each)
+    const unsigned nt = reg2 ? (unsigned)(L.G << t2_r1log(L.logT)) : 256u;
+    if (nt > 512 || nt == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k, grid, dim3(nt), lds, stream, prm);
     return hipGetLastError();
 }

@@ -12,7 +12,7 @@ struct ArF64 {
     using TW = double2;        // twiddle (w, w/p)
     using PC = F64PrimeDev;
     using NI = double;         // element type of (N^-1, aux) pairs
-    __device__ static V from_word(W x, const PC&) { return (double)x; }
+    __device__ static V from_word(W x, const PC&) { return (double)(int32_t)x; }   // canonical or centered
     __device__ static V from_raw(Raw x, const PC&) { return (double)x; }   // < 2^32: no reduction needed
     __device__ static W to_word(V x, const PC& c) { return f64_canon(x, c.P); }
     __device__ static V add(V a, V b, const PC&) { return a + b; }
@@ -65,7 +65,7 @@ __device__ __forceinline__ void dft_dif(typename AR::V* x, const typename AR::PC
                 auto a = x[blk + i], b = x[blk + i + h];
                 x[blk + i] = AR::add(a, b, pc);
                 auto d = AR::sub(a, b, pc);
-                x[blk + i + h] = (i == 0) ? d : AR::mulc(d, pc.w16[i * (8 / h)], pc);
+                x[blk + i + h] = (i == 0) ? d : AR::mulc(d, pc.w32[i * (16 / h)], pc);
             }
         }
     }
@@ -84,7 +84,7 @@ __device__ __forceinline__ void dft_dit_inv(typename AR::V* x, const typename AR
 #pragma unroll
             for (int i = 0; i < h; i++) {
                 auto a = x[blk + i], b = x[blk + i + h];
-                if (i != 0) b = AR::mulc(b, pc.iw16[i * (8 / h)], pc);
+                if (i != 0) b = AR::mulc(b, pc.iw32[i * (16 / h)], pc);
                 x[blk + i] = AR::add(a, b, pc);
                 x[blk + i + h] = AR::sub(a, b, pc);
             }

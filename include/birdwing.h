@@ -45,6 +45,9 @@ enum {
     BW_ERR_STATE = -5
 };
 
+/* Launch order of a plan's passes (bw_plan_set_schedule). */
+enum { BW_SCHED_PASS_MAJOR = 0, BW_SCHED_PRIME_MAJOR = 1 };
+
 enum { BW_SET_DEFAULT = -1, BW_SET_U32_REF = 0, BW_SET_U32_WIDE = 1, BW_SET_U64_REF = 2, BW_SET_HYBRID = 3 };
 
 typedef struct bw_plan bw_plan;
@@ -103,6 +106,10 @@ int bw_execute(bw_plan* plan, const void* d_a, const void* d_b, void* d_c, void*
  * 64-bit path: c0 = lo, c1 = mid, c2 = hi 32 bits (U160). */
 int bw_execute_crt(bw_plan* plan, const void* d_a, const void* d_b, uint64_t* d_c0, uint64_t* d_c1,
                    uint32_t* d_c2, void* stream);
+/* Launch order: BW_SCHED_PRIME_MAJOR (default) finishes one prime's chain of
+ * passes before the next so intermediates are re-read from the Infinity Cache;
+ * BW_SCHED_PASS_MAJOR issues one launch per pass over all primes. */
+int bw_plan_set_schedule(bw_plan* plan, int schedule);
 /* Synchronizes and reports BW_ERR_STATE if a carry look-back wait of this plan
  * ever exceeded its bound (a failure that would otherwise be a hang). */
 int bw_plan_status(bw_plan* plan);

@@ -325,6 +325,22 @@ def test_product_pass_splits(gpu, oracle, passes):
     _product_check(gpu, oracle, A, B, 64, -1, passes)
 
 
+@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("bits,L", [(32, 3000), (32, 1 << 18), (64, 1 << 16)])
+def test_both_schedules(gpu, oracle, bits, L, schedule):
+    """Pass-major and prime-major launch orders give the same product."""
+    d = dev()
+    A = oracle.uniform_limbs(L, 5, bits)
+    B = oracle.uniform_limbs(L - 7, 6, bits)
+    plan = gpu.Plan(bits, len(A), len(B))
+    plan.set_schedule(schedule)
+    c = d.empty(len(A) + len(B), bits)
+    plan.execute(d.to_dev(A), d.to_dev(B), c)
+    want, _ = oracle.gmp_mul(A, B, bits)
+    assert np.array_equal(d.from_dev(c, len(A) + len(B)), want)
+    plan.status()
+
+
 @pytest.mark.parametrize("bits,L", [(32, 4), (32, 256), (32, 1 << 15), (64, 64), (64, 4096)])
 def test_execute_crt_matches_oracle(gpu, oracle, bits, L):
     """Early exit after Garner (src/gpu_ntt.cu:843-884; tests/test_pipeline_crt.cpp:209-259)."""
