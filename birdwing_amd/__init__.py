@@ -26,6 +26,7 @@ __all__ = [
     "padded_ntt_size", "multiply_size_supported", "ntt_size_supported", "crt_bound_satisfied",
     "max_supported_logN", "max_supported_limb_count", "max_limb_count_by_crt", "default_prime_set",
     "prime_set_info", "crt_combine", "carry128", "carry160", "zero_pad", "zero_pad_u64", "pointwise",
+    "Batch", "batch_footprint", "tune_count", "tune_clear",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -48,7 +49,14 @@ class PlanInfo(C.Structure):
                 ("pass_log", C.c_int * 4), ("nlaunch", C.c_int), ("device_bytes", C.c_uint64)]
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [("n", C.c_int), ("ngroups", C.c_int), ("nlaunch", C.c_int), ("use_graph", C.c_int),
+                ("bytes_a", C.c_uint64), ("bytes_b", C.c_uint64), ("bytes_c", C.c_uint64),
+                ("workspace_bytes", C.c_uint64)]
+
+
 _vp, _u64, _i, _sz, _cp = C.c_void_p, C.c_uint64, C.c_int, C.c_size_t, C.c_char_p
+_pi, _pu64 = C.POINTER(C.c_int), C.POINTER(C.c_uint64)
 # name -> (restype, argtypes); every symbol declared in include/birdwing.h
 SIGNATURES = {
     "bw_last_error": (_cp, []),
@@ -84,6 +92,18 @@ SIGNATURES = {
     "bw_crt_combine": (_i, [_i, _vp, _vp, _vp, _u64, _vp, _vp, _vp, _vp]),
     "bw_carry128": (_i, [_vp, _vp, _u64, _vp, _u64, _i, _vp]),
     "bw_carry160": (_i, [_vp, _vp, _vp, _u64, _vp, _u64, _vp]),
+    "bw_batch_create": (_i, [C.POINTER(_vp), _i, _pi, _pu64, _pu64]),
+    "bw_batch_destroy": (None, [_vp]),
+    "bw_batch_footprint": (_i, [_i, _pi, _pu64, _pu64, _pu64, _pu64, _pu64, _pu64]),
+    "bw_batch_get_info": (_i, [_vp, C.POINTER(BatchInfo)]),
+    "bw_batch_offsets": (_i, [_vp, _pu64, _pu64, _pu64]),
+    "bw_batch_group_info": (_i, [_vp, _i, _pi, _pi, _pi, _pi, _pi, _pi, _pi]),
+    "bw_batch_execute": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "bw_batch_set_graph": (_i, [_vp, _i]),
+    "bw_batch_tune": (_i, [_vp, _vp, _vp, _vp, _i]),
+    "bw_tune_count": (_i, []),
+    "bw_tune_clear": (None, []),
+    "bw_batch_status": (_i, [_vp]),
 }
 
 _LIB = None
@@ -291,3 +311,112 @@ def carry128(hi, lo, N, out, n_out, out_bits=32, stream=None):
 
 def carry160(lo, mid, hi, N, out, n_out, stream=None):
     _check(lib().bw_carry160(_ptr(lo), _ptr(mid), _ptr(hi), N, _ptr(out), n_out, _stream(stream)))
+
+
+# ---- batched products ---------------------------------------------------------
+class Batch:
+    """Many products of uneven sizes (limb_bits[i] in {32, 64}) run together.
+
+    Operands and results live in packed byte buffers: product i reads its A
+    limbs at byte offset off_a[i] of the packed A buffer, and so on.  pack() /
+    unpack() move between lists of numpy limb arrays and packed host buffers.
+    """
+
+    def __init__(self, limb_bits, L_A, L_B):
+        n = len(L_A)
+        if isinstance(limb_bits, int):
+            limb_bits = [limb_bits] * n
+        self.n = n
+        self.bits = [int(b) for b in limb_bits]
+        self.L_A = [int(x) for x in L_A]
+        self.L_B = [int(x) for x in L_B]
+        self._b = C.c_void_p()
+        bits = (C.c_int * n)(*self.bits)
+        la = (C.c_uint64 * n)(*self.L_A)
+        lb = (C.c_uint64 * n)(*self.L_B)
+        _check(lib().bw_batch_create(C.byref(self._b), n, bits, la, lb))
+        self.info = BatchInfo()
+        _check(lib().bw_batch_get_info(self._b, C.byref(self.info)))
+        oa, ob, oc = (C.c_uint64 * n)(), (C.c_uint64 * n)(), (C.c_uint64 * n)()
+        _check(lib().bw_batch_offsets(self._b, oa, ob, oc))
+        self.off_a = [int(x) for x in oa]
+        self.off_b = [int(x) for x in ob]
+        self.off_c = [int(x) for x in oc]
+        self.bytes_a, self.bytes_b, self.bytes_c = int(self.info.bytes_a), int(self.info.bytes_b), int(self.info.bytes_c)
+
+    def close(self):
+        if self._b:
+            lib().bw_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def groups(self):
+        out = []
+        for gi in range(int(self.info.ngroups)):
+            v = [C.c_int() for _ in range(5)]
+            pl = (C.c_int * 4)()
+            npass, gsh = C.c_int(), C.c_int()
+            _check(lib().bw_batch_group_info(self._b, gi, C.byref(v[0]), C.byref(v[1]), C.byref(v[2]),
+                                             C.byref(v[3]), pl, C.byref(npass), C.byref(gsh)))
+            out.append({"limb_bits": v[0].value, "logN": v[1].value, "members": v[2].value, "slots": v[3].value,
+                        "passes": [pl[j] for j in range(npass.value)], "gshift": gsh.value})
+        return out
+
+    def pack(self, As, Bs):
+        """Lists of limb arrays -> packed host buffers (numpy uint8)."""
+        pa = np.zeros(max(self.bytes_a, 1), dtype=np.uint8)
+        pb = np.zeros(max(self.bytes_b, 1), dtype=np.uint8)
+        for i in range(self.n):
+            dt = _dt(self.bits[i])
+            a = np.ascontiguousarray(As[i], dtype=dt).view(np.uint8)
+            b = np.ascontiguousarray(Bs[i], dtype=dt).view(np.uint8)
+            pa[self.off_a[i]:self.off_a[i] + a.size] = a
+            pb[self.off_b[i]:self.off_b[i] + b.size] = b
+        return pa, pb
+
+    def unpack(self, pc):
+        """Packed result bytes -> list of product limb arrays."""
+        pc = np.asarray(pc).view(np.uint8)
+        out = []
+        for i in range(self.n):
+            dt = _dt(self.bits[i])
+            nb = (self.L_A[i] + self.L_B[i]) * (self.bits[i] // 8)
+            out.append(pc[self.off_c[i]:self.off_c[i] + nb].view(dt).copy())
+        return out
+
+    def execute(self, a, b, c, stream=None):
+        """Packed device buffers (torch uint8 tensors) -> packed results in c."""
+        _check(lib().bw_batch_execute(self._b, _ptr(a), _ptr(b), _ptr(c), _stream(stream)))
+
+    def set_graph(self, on):
+        _check(lib().bw_batch_set_graph(self._b, 1 if on else 0))
+
+    def tune(self, a, b, stream=None, reps=3):
+        _check(lib().bw_batch_tune(self._b, _ptr(a), _ptr(b), _stream(stream), reps))
+
+    def status(self):
+        _check(lib().bw_batch_status(self._b))
+
+
+def batch_footprint(limb_bits, L_A, L_B):
+    """Host-only: (workspace, packed A, packed B, packed C) bytes of a batch."""
+    n = len(L_A)
+    if isinstance(limb_bits, int):
+        limb_bits = [limb_bits] * n
+    out = [C.c_uint64() for _ in range(4)]
+    _check(lib().bw_batch_footprint(n, (C.c_int * n)(*limb_bits), (C.c_uint64 * n)(*L_A), (C.c_uint64 * n)(*L_B),
+                                    *[C.byref(o) for o in out]))
+    return tuple(int(o.value) for o in out)
+
+
+def tune_count():
+    return int(lib().bw_tune_count())
+
+
+def tune_clear():
+    lib().bw_tune_clear()

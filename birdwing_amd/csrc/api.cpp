@@ -26,75 +26,14 @@
 #include <vector>
 
 #include "../../include/birdwing.h"
-#include "bw_internal.h"
-#include "carry_kernels.h"
-#include "host_math.h"
+#include "engine.h"
 #include "misc_kernels.h"
-#include "ntt_kernels.h"
 
 using namespace bw;
 
-// ---------------------------------------------------------------------------
-// errors
-static thread_local std::string g_err;
-
-static int fail(int code, const char* fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    g_err = buf;
-    return code;
-}
-
-#define HIP_TRY(expr)                                                                                   \
-    do {                                                                                                \
-        hipError_t e_ = (expr);                                                                         \
-        if (e_ != hipSuccess) return fail(BW_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
-
-extern "C" const char* bw_last_error(void) { return g_err.c_str(); }
+namespace bw { const char* last_error(); }
+extern "C" const char* bw_last_error(void) { return bw::last_error(); }
 extern "C" const char* bw_version(void) { return "birdwing-mi355x 0.1 (gfx950)"; }
-
-// ---------------------------------------------------------------------------
-// per (device, set) tables
-struct DevTables {
-    SetTables t;
-    GarnerDev garner;
-    F64Prime fp[3];
-};
-
-static std::mutex g_tab_mu;
-static std::map<std::pair<int, int>, std::unique_ptr<DevTables>> g_tabs;
-
-static int get_tables(int set_id, DevTables** out) {
-    int dev = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lk(g_tab_mu);
-    auto key = std::make_pair(dev, set_id);
-    auto it = g_tabs.find(key);
-    if (it != g_tabs.end()) { *out = it->second.get(); return BW_OK; }
-    const PrimeSet* s = prime_set(set_id);
-    if (!s) return fail(BW_ERR_ARG, "unknown prime set %d", set_id);
-    HostTables h;
-    build_tables(*s, h);
-    std::unique_ptr<DevTables> d(new DevTables());
-    d->t.limb_bits = s->limb_bits;
-    d->t.k = s->k;
-    d->t.table_log = h.table_log;
-    void** ptrs[4] = {&d->t.consts, &d->t.tw_stage, &d->t.tw_lo, &d->t.tw_hi};
-    std::vector<unsigned char>* srcs[4] = {&h.consts, &h.tw_stage, &h.tw_lo, &h.tw_hi};
-    for (int i = 0; i < 4; i++) {
-        HIP_TRY(hipMalloc(ptrs[i], srcs[i]->size()));
-        HIP_TRY(hipMemcpy(*ptrs[i], srcs[i]->data(), srcs[i]->size(), hipMemcpyHostToDevice));
-    }
-    d->garner = garner_constants(*s);
-    for (int i = 0; i < 3; i++) f64_prime(i < s->k ? s->p[i] : 3, d->fp[i]);
-    *out = d.get();
-    g_tabs[key] = std::move(d);
-    return BW_OK;
-}
 
 extern "C" int bw_init(int device) {
     HIP_TRY(hipSetDevice(device));
@@ -199,15 +138,6 @@ struct bw_plan {
     int nev = 0;               // events recorded by the current execution
 };
 
-// Default pass split: passes of at most 2^9 points (register-staged kernels,
-// column tiles of 16+ adjacent columns), balanced, the largest share last.
-static void default_passes(int logN, std::vector<int>& out) {
-    out.clear();
-    const int P = (logN + 8) / 9 < 1 ? 1 : (logN + 8) / 9;
-    const int base = logN / P, extra = logN % P;
-    for (int j = 0; j < P; j++) out.push_back(base + (j >= P - extra ? 1 : 0));
-}
-
 static void plan_free(bw_plan* p) {
     if (!p) return;
     (void)hipFree(p->bufA);
@@ -311,12 +241,23 @@ extern "C" int bw_plan_create(bw_plan** out, int limb_bits, uint64_t L_A, uint64
 
 extern "C" void bw_plan_destroy(bw_plan* p) { plan_free(p); }
 
-static int count_launches(const bw_plan* p) {
-    const int P = (int)p->passes.size();
-    if (P == 1) return 2;
-    if (p->schedule == BW_SCHED_PRIME_MAJOR) return p->set->k * (2 * (P - 1) + 1 + (P - 1)) + 1;
-    return (P - 1) + 1 + (P - 1) + 1;
+static Xform plan_xform(const bw_plan* p) {
+    Xform x;
+    x.limb_bits = p->limb_bits;
+    x.set = p->set;
+    x.tabs = p->tabs;
+    x.logN = p->logN;
+    x.N = (int64_t)p->N_exec;
+    x.B = 1;
+    x.passes = p->passes;
+    x.bufA = p->bufA;
+    x.bufB = p->bufB;
+    x.ninv = p->ninv;
+    x.schedule = p->schedule;
+    return x;
 }
+
+static int count_launches(const bw_plan* p) { return count_transform_launches(plan_xform(p)) + 1; }
 
 extern "C" int bw_plan_get_info(const bw_plan* p, bw_plan_info* info) {
     if (!p || !info) return fail(BW_ERR_ARG, "NULL argument");
@@ -338,70 +279,12 @@ extern "C" int bw_plan_get_info(const bw_plan* p, bw_plan_info* info) {
 
 // ---------------------------------------------------------------------------
 // execution
-struct PassPlan {
-    int logT;
-    int64_t S, O;
-    int logM;
-    int G;
-    int64_t tiles;
-};
-
-static PassPlan pass_geom(const bw_plan* p, int j) {
-    PassPlan g;
-    int prefix = 0;
-    for (int i = 0; i < j; i++) prefix += p->passes[i];
-    g.logT = p->passes[j];
-    g.logM = p->logN - prefix;
-    g.S = (int64_t)1 << (p->logN - prefix - g.logT);
-    g.O = (int64_t)1 << prefix;
-    int gmax = bw_pass_max_G(g.logT);
-    if (g.S > 1) {
-        g.G = (int)(g.S < gmax ? g.S : gmax);
-    } else {
-        // rows: G * T <= 4096 keeps the fused kernel's per-thread registers small
-        while (gmax > 1 && ((int64_t)gmax << g.logT) > 4096) gmax >>= 1;
-        g.G = (int)(g.O < gmax ? g.O : gmax);
-    }
-    g.tiles = g.O * g.S / g.G;
-    return g;
-}
-
-static PassParams base_params(const bw_plan* p, const PassPlan& g) {
-    PassParams prm;
-    std::memset(&prm, 0, sizeof(prm));
-    prm.N = (int64_t)p->N_exec;
-    prm.plane = (int64_t)p->N_exec;
-    prm.S = g.S;
-    prm.O = g.O;
-    prm.logM = g.logM;
-    prm.table_log = p->tabs->t.table_log;
-    prm.consts = p->tabs->t.consts;
-    prm.tw_stage = p->tabs->t.tw_stage;
-    prm.tw_lo = p->tabs->t.tw_lo;
-    prm.tw_hi = p->tabs->t.tw_hi;
-    prm.ninv = p->ninv;
-    prm.canon_out = 1;
-    return prm;
-}
-
-static hipError_t launch(const bw_plan* p, int mode, const PassPlan& g, const PassParams& prm, int nops,
-                         hipStream_t st) {
-    PassLaunch L;
-    L.mode = mode;
-    L.logT = g.logT;
-    L.G = g.G;
-    L.limb_bits = p->limb_bits;
-    L.nprimes = p->set->k;
-    L.nops = nops;
-    L.tiles = g.tiles;
-    return bw_launch_pass(L, prm, st);
-}
-
-struct Marker {
+struct Marker : LaunchMarker {
     bw_plan* p;
     hipStream_t st;
     int n = 0;
-    hipError_t mark() {
+    Marker(bw_plan* p_, hipStream_t st_) : p(p_), st(st_) {}
+    hipError_t mark() override {
         if (!p->timing || n > BW_MAX_LAUNCH) return hipSuccess;
         if (n == 0) {                                  // first mark of an execution: take an event set
             if (p->nexec == p->evpool.size()) {
@@ -421,92 +304,8 @@ struct Marker {
     }
 };
 
-#define LAUNCH_TRY(expr)                                                                                 \
-    do {                                                                                                 \
-        hipError_t e_ = (expr);                                                                          \
-        if (e_ != hipSuccess) return fail(BW_ERR_HIP, "kernel launch failed: %s", hipGetErrorString(e_)); \
-    } while (0)
-
-// Transform + product + inverse, leaving canonical residues in bufA.
-//
-// BW_SCHED_PASS_MAJOR: one launch per pass over every (prime, operand).
-// BW_SCHED_PRIME_MAJOR: the chain of one (prime, operand) -- forward passes of
-// A_p, of B_p, the fused pass, the inverse passes of prime p -- runs before the
-// next prime starts, so each 128 MiB intermediate (at N = 2^25) is re-read
-// while it is still held by the 256 MB Infinity Cache.
-static int run_transforms(bw_plan* p, const void* d_a, const void* d_b, hipStream_t st, Marker& mk) {
-    const int P = (int)p->passes.size();
-    if (P == 1) {
-        PassPlan g = pass_geom(p, 0);
-        PassParams prm = base_params(p, g);
-        prm.ingress = 1;
-        prm.in[0] = d_a;
-        prm.in[1] = d_b;
-        prm.in_len[0] = (int64_t)p->L_A;
-        prm.in_len[1] = (int64_t)p->L_B;
-        prm.out[0] = p->bufA;
-        LAUNCH_TRY(launch(p, BW_MODE_FUSED, g, prm, 1, st));
-        LAUNCH_TRY(mk.mark());
-        return BW_OK;
-    }
-    const bool folded = pass_geom(p, P - 1).logT <= BW_REG2_MAX_LOG && pass_geom(p, P - 2).logT <= BW_REG2_MAX_LOG;
-    const bool pm = p->schedule == BW_SCHED_PRIME_MAJOR;
-    const int k = p->set->k;
-    const int nchunk = pm ? k : 1;          // prime chunks
-    const int nper = pm ? 1 : k;            // primes per launch
-    const void* raw[2] = {d_a, d_b};
-    void* buf[2] = {p->bufA, p->bufB};
-    const int64_t len[2] = {(int64_t)p->L_A, (int64_t)p->L_B};
-    for (int c = 0; c < nchunk; c++) {
-        const int prime0 = pm ? c : 0;
-        // forward passes 1 .. P-1: per operand (prime-major) or both operands at once
-        for (int op = 0; op < (pm ? 2 : 1); op++) {
-            for (int j = 0; j < P - 1; j++) {
-                PassPlan g = pass_geom(p, j);
-                PassParams prm = base_params(p, g);
-                prm.prime0 = prime0;
-                prm.canon_out = 0;
-                const int nops = pm ? 1 : 2;
-                for (int q = 0; q < nops; q++) {
-                    const int which = pm ? op : q;
-                    prm.in[q] = j == 0 ? raw[which] : (const void*)buf[which];
-                    prm.in_len[q] = len[which];
-                    prm.out[q] = buf[which];
-                }
-                prm.ingress = j == 0;
-                prm.nops = nops;
-                PassLaunch L = {BW_MODE_FWD, g.logT, g.G, p->limb_bits, nper, nops, g.tiles};
-                LAUNCH_TRY(bw_launch_pass(L, prm, st));
-                LAUNCH_TRY(mk.mark());
-            }
-        }
-        {
-            PassPlan g = pass_geom(p, P - 1);
-            PassParams prm = base_params(p, g);
-            prm.prime0 = prime0;
-            prm.in[0] = p->bufA;
-            prm.in[1] = p->bufB;
-            prm.out[0] = p->bufA;
-            prm.canon_out = 0;
-            prm.fold_ninv = folded;
-            PassLaunch L = {BW_MODE_FUSED, g.logT, g.G, p->limb_bits, nper, 1, g.tiles};
-            LAUNCH_TRY(bw_launch_pass(L, prm, st));
-            LAUNCH_TRY(mk.mark());
-        }
-        for (int j = P - 2; j >= 0; j--) {
-            PassPlan g = pass_geom(p, j);
-            PassParams prm = base_params(p, g);
-            prm.prime0 = prime0;
-            prm.in[0] = p->bufA;
-            prm.out[0] = p->bufA;
-            prm.canon_out = j == 0 ? 1 : 0;
-            prm.fold_ninv = (j == P - 2) && folded;
-            PassLaunch L = {BW_MODE_INV, g.logT, g.G, p->limb_bits, nper, 1, g.tiles};
-            LAUNCH_TRY(bw_launch_pass(L, prm, st));
-            LAUNCH_TRY(mk.mark());
-        }
-    }
-    return BW_OK;
+static int run_plan_transforms(bw_plan* p, const void* d_a, const void* d_b, hipStream_t st, Marker& mk) {
+    return run_transforms(plan_xform(p), d_a, d_b, (int64_t)p->L_A, (int64_t)p->L_B, st, &mk);
 }
 
 static int check_plan(const bw_plan* p) {
@@ -519,10 +318,10 @@ extern "C" int bw_execute(bw_plan* p, const void* d_a, const void* d_b, void* d_
     if (rc) return rc;
     if (!d_a || !d_b || !d_c) return fail(BW_ERR_ARG, "NULL device buffer");
     hipStream_t st = (hipStream_t)stream;
-    Marker mk{p, st};
+    Marker mk(p, st);
     p->nev = 0;
     LAUNCH_TRY(mk.mark());
-    if ((rc = run_transforms(p, d_a, d_b, st, mk))) return rc;
+    if ((rc = run_plan_transforms(p, d_a, d_b, st, mk))) return rc;
     CarryParams cp;
     std::memset(&cp, 0, sizeof(cp));
     cp.src = p->limb_bits == 64 ? BW_SRC_GARNER_U64 : BW_SRC_GARNER_U32;
@@ -549,10 +348,10 @@ extern "C" int bw_execute_crt(bw_plan* p, const void* d_a, const void* d_b, uint
     if (rc) return rc;
     if (!d_a || !d_b || !d_c0 || !d_c1 || (p->limb_bits == 64 && !d_c2)) return fail(BW_ERR_ARG, "NULL device buffer");
     hipStream_t st = (hipStream_t)stream;
-    Marker mk{p, st};
+    Marker mk(p, st);
     p->nev = 0;
     LAUNCH_TRY(mk.mark());
-    if ((rc = run_transforms(p, d_a, d_b, st, mk))) return rc;
+    if ((rc = run_plan_transforms(p, d_a, d_b, st, mk))) return rc;
     const int64_t N = (int64_t)p->N_exec;
     const unsigned char* base = (const unsigned char*)p->bufA;
     LAUNCH_TRY(bw_launch_garner_only(p->limb_bits, base, base + N * p->word, base + 2 * N * p->word, N,
@@ -653,10 +452,11 @@ extern "C" int bw_forward(bw_plan* p, const void* d_raw, uint64_t L, void* d_spe
     if (!d_raw || !d_spec) return fail(BW_ERR_ARG, "NULL device buffer");
     hipStream_t st = (hipStream_t)stream;
     const int P = (int)p->passes.size();
+    const Xform x = plan_xform(p);
     for (int j = 0; j < P; j++) {
-        PassPlan g = pass_geom(p, j);
+        PassPlan g = pass_geom(x, j);
         if (j == P - 1 && g.S == 1 && P == 1) g.G = 1;
-        PassParams prm = base_params(p, g);
+        PassParams prm = base_params(x, g);
         prm.nops = 1;
         if (j == 0) {
             prm.ingress = 1;
@@ -666,7 +466,7 @@ extern "C" int bw_forward(bw_plan* p, const void* d_raw, uint64_t L, void* d_spe
             prm.in[0] = d_spec;
         }
         prm.out[0] = d_spec;
-        LAUNCH_TRY(launch(p, BW_MODE_FWD, g, prm, 1, st));
+        LAUNCH_TRY(launch_pass(x, BW_MODE_FWD, g, prm, 1, st));
     }
     return BW_OK;
 }
@@ -677,12 +477,13 @@ extern "C" int bw_inverse(bw_plan* p, void* d_spec, void* stream) {
     if (!d_spec) return fail(BW_ERR_ARG, "NULL device buffer");
     hipStream_t st = (hipStream_t)stream;
     const int P = (int)p->passes.size();
+    const Xform x = plan_xform(p);
     for (int j = P - 1; j >= 0; j--) {
-        PassPlan g = pass_geom(p, j);
-        PassParams prm = base_params(p, g);
+        PassPlan g = pass_geom(x, j);
+        PassParams prm = base_params(x, g);
         prm.in[0] = d_spec;
         prm.out[0] = d_spec;
-        LAUNCH_TRY(launch(p, BW_MODE_INV, g, prm, 1, st));
+        LAUNCH_TRY(launch_pass(x, BW_MODE_INV, g, prm, 1, st));
     }
     LAUNCH_TRY(bw_launch_scale(p->limb_bits, d_spec, (int64_t)p->N_exec, (int64_t)p->N_exec, p->set->k,
                                p->tabs->t.consts, p->ninv, st));

@@ -59,6 +59,14 @@ struct PassGeom {
 
 enum { BW_MODE_FWD = 0, BW_MODE_INV = 1, BW_MODE_FUSED = 2 };
 
+// Batched ingress: products stacked along the outer dimension of the first
+// pass (product o owns elements [o N, (o+1) N) of every prime plane) read their
+// raw limbs at in[0] + a_off (a_len limbs) and in[1] + b_off (b_len limbs).
+struct BwIngress {
+    int64_t a_off, b_off;
+    int64_t a_len, b_len;
+};
+
 struct PassParams {
     // operands: FWD handles nops operands (A then B); FUSED reads both, writes out0;
     // INV reads/writes operand 0.
@@ -80,6 +88,7 @@ struct PassParams {
     const void* tw_lo;
     const void* tw_hi;
     const void* ninv;      // per-prime (N^-1, aux) pairs for FUSED mode
+    const BwIngress* bdesc;  // ingress of stacked products (null: one product, in_len)
 };
 
 // Status codes (BW_OK, BW_ERR_*) come from the public header.

@@ -24,6 +24,18 @@ enum {
     BW_SRC_HILO128_L64 = 4,  // (hi, lo) u64 arrays -> 64-bit limbs
 };
 
+// Batched carry: one launch covers many products; product j owns tiles
+// [tile0, tile0 + ceil(n_out / tile)) of the launch and reads its 3 residue
+// planes at res + i * plane (i = prime), N coefficients each.
+struct BwCarryJob {
+    const void* res;
+    int64_t plane;
+    int64_t N;
+    int64_t n_out;
+    int64_t out_off;      // limbs from CarryParams::out
+    int64_t tile0;
+};
+
 struct CarryParams {
     int src;
     const void* a0;       // residues base / hi / lo arrays (see src)
@@ -40,10 +52,15 @@ struct CarryParams {
     uint32_t epoch;       // tags this launch's flags
     int64_t ntiles;
     int* err;             // set to 1 if a look-back wait exceeded its bound
+    const BwCarryJob* jobs;        // batched launch (null: the single product above)
+    const int* tile_job;           // batched: product of each tile
 };
 
 hipError_t bw_launch_garner_carry(const CarryParams& p, hipStream_t stream);
 int64_t bw_carry_tiles(int64_t n_out);
+// Clears the tile flags and the tile-id counter of a carry launch (a kernel,
+// so that the sequence can live in a hipGraph).
+hipError_t bw_launch_carry_reset(unsigned long long* flags, int64_t ntiles, int* counter, hipStream_t stream);
 // Standalone Garner (coefficients out): u32 path -> (hi, lo); u64 path -> U160.
 hipError_t bw_launch_garner_only(int limb_bits, const void* r0, const void* r1, const void* r2, int64_t N,
                                  const GarnerDev& g, const F64Prime* fp,

@@ -48,30 +48,38 @@ __device__ __forceinline__ void garner_u64(uint64_t r0, uint64_t r1, uint64_t r2
 
 // ---------------------------------------------------------------------------
 // Coefficient sources: words w_j(c_n) in limb width, zero outside [0, N).
+// The view names the coefficient arrays of the product a tile belongs to.
+struct CarryView {
+    const void* a0;
+    const void* a1;
+    const void* a2;
+    int64_t plane;
+    int64_t N;
+};
 struct SrcGarnerU32 {
     using Word = uint32_t;
     static constexpr int NW = 3;
-    __device__ static void load(const CarryParams& p, int64_t n, Word* w) {
+    __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
         if (n < 0 || n >= p.N) { w[0] = w[1] = w[2] = 0; return; }
         const uint32_t* r = reinterpret_cast<const uint32_t*>(p.a0);
         uint64_t lo; uint32_t hi;
-        garner_u32(r[n], r[p.plane + n], r[2 * p.plane + n], p.g, p.fp, lo, hi);
+        garner_u32(r[n], r[p.plane + n], r[2 * p.plane + n], c.g, c.fp, lo, hi);
         w[0] = (uint32_t)lo; w[1] = (uint32_t)(lo >> 32); w[2] = hi;
     }
 };
 struct SrcGarnerU64 {
     using Word = uint64_t;
     static constexpr int NW = 3;
-    __device__ static void load(const CarryParams& p, int64_t n, Word* w) {
+    __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
         if (n < 0 || n >= p.N) { w[0] = w[1] = w[2] = 0; return; }
         const uint64_t* r = reinterpret_cast<const uint64_t*>(p.a0);
-        garner_u64(r[n], r[p.plane + n], r[2 * p.plane + n], p.g, w[0], w[1], w[2]);
+        garner_u64(r[n], r[p.plane + n], r[2 * p.plane + n], c.g, w[0], w[1], w[2]);
     }
 };
 struct SrcHiLo128 {     // a0 = hi, a1 = lo
     using Word = uint32_t;
     static constexpr int NW = 4;
-    __device__ static void load(const CarryParams& p, int64_t n, Word* w) {
+    __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
         if (n < 0 || n >= p.N) { w[0] = w[1] = w[2] = w[3] = 0; return; }
         const uint64_t hi = reinterpret_cast<const uint64_t*>(p.a0)[n];
         const uint64_t lo = reinterpret_cast<const uint64_t*>(p.a1)[n];
@@ -81,7 +89,7 @@ struct SrcHiLo128 {     // a0 = hi, a1 = lo
 struct SrcHiLo128L64 {  // a0 = hi, a1 = lo, 64-bit limbs
     using Word = uint64_t;
     static constexpr int NW = 2;
-    __device__ static void load(const CarryParams& p, int64_t n, Word* w) {
+    __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
         if (n < 0 || n >= p.N) { w[0] = w[1] = 0; return; }
         w[1] = reinterpret_cast<const uint64_t*>(p.a0)[n];
         w[0] = reinterpret_cast<const uint64_t*>(p.a1)[n];
@@ -90,7 +98,7 @@ struct SrcHiLo128L64 {  // a0 = hi, a1 = lo, 64-bit limbs
 struct SrcU160 {        // a0 = lo, a1 = mid, a2 = hi32
     using Word = uint64_t;
     static constexpr int NW = 3;
-    __device__ static void load(const CarryParams& p, int64_t n, Word* w) {
+    __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
         if (n < 0 || n >= p.N) { w[0] = w[1] = w[2] = 0; return; }
         w[0] = reinterpret_cast<const uint64_t*>(p.a0)[n];
         w[1] = reinterpret_cast<const uint64_t*>(p.a1)[n];
@@ -129,7 +137,21 @@ __global__ __launch_bounds__(256) void garner_carry_kernel(CarryParams prm) {
     const int64_t tile = s_tile;
     const int wave = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
-    const int64_t wbase = tile * BW_CARRY_TILE + (int64_t)wave * 64 * K;
+    // the product this tile belongs to, and the tile's index inside it
+    CarryView view = {prm.a0, prm.a1, prm.a2, prm.plane, prm.N};
+    int64_t n_out = prm.n_out;
+    Wd* out = reinterpret_cast<Wd*>(prm.out);
+    int64_t ltile = tile;
+    if (prm.jobs) {
+        const BwCarryJob J = prm.jobs[prm.tile_job[tile]];
+        view.a0 = J.res;
+        view.plane = J.plane;
+        view.N = J.N;
+        n_out = J.n_out;
+        out += J.out_off;
+        ltile = tile - J.tile0;
+    }
+    const int64_t wbase = ltile * BW_CARRY_TILE + (int64_t)wave * 64 * K;
 
     Wd tval[K];
     bool tg[K], tp[K];
@@ -138,7 +160,7 @@ __global__ __launch_bounds__(256) void garner_carry_kernel(CarryParams prm) {
     // Row -1 supplies the lower neighbours of row 0 (its own lanes < NW-1 are never used).
     {
         Wd cw[NW];
-        SRC::load(prm, wbase - 64 + lane, cw);
+        SRC::load(view, prm, wbase - 64 + lane, cw);
         Wd zero[NW];
 #pragma unroll
         for (int j = 0; j < NW; j++) zero[j] = 0;
@@ -158,7 +180,7 @@ __global__ __launch_bounds__(256) void garner_carry_kernel(CarryParams prm) {
 #pragma unroll
     for (int i = 0; i < K; i++) {
         Wd cw[NW];
-        SRC::load(prm, wbase + i * 64 + lane, cw);
+        SRC::load(view, prm, wbase + i * 64 + lane, cw);
         Wd lo = cw[0];
         Wd hi = 0;
 #pragma unroll
@@ -201,9 +223,10 @@ __global__ __launch_bounds__(256) void garner_carry_kernel(CarryParams prm) {
             gen = (a & 1) | ((a >> 1) & gen);
             prop &= (a >> 1);
         }
-        const unsigned long long ep = (unsigned long long)prm.epoch << 32;
+        const uint32_t epoch = prm.epoch;
+        const unsigned long long ep = (unsigned long long)epoch << 32;
         uint32_t cin = 0;
-        if (tile == 0) {
+        if (ltile == 0) {
             __hip_atomic_store(&prm.flags[tile], ep | FLAG_INC | ((unsigned long long)gen << 2),
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
@@ -216,7 +239,7 @@ __global__ __launch_bounds__(256) void garner_carry_kernel(CarryParams prm) {
             for (;;) {
                 const unsigned long long f =
                     __hip_atomic_load(&prm.flags[pred], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if ((uint32_t)(f >> 32) != prm.epoch || (f & 3ULL) == 0) {
+                if ((uint32_t)(f >> 32) != epoch || (f & 3ULL) == 0) {
                     // Predecessors publish their summary without waiting, so this
                     // wait is short; the bound only turns a logic error into a
                     // reported failure instead of a hang.
@@ -253,7 +276,6 @@ __global__ __launch_bounds__(256) void garner_carry_kernel(CarryParams prm) {
     __syncthreads();
 
     uint64_t cin = s_wcin[wave];
-    Wd* out = reinterpret_cast<Wd*>(prm.out);
 #pragma unroll
     for (int i = 0; i < K; i++) {
         const uint64_t G = __ballot(tg[i]);
@@ -265,12 +287,26 @@ __global__ __launch_bounds__(256) void garner_carry_kernel(CarryParams prm) {
         const uint64_t carries = s2 ^ P;           // carry into each lane
         const Wd limb = tval[i] + (Wd)((carries >> lane) & 1ULL);
         const int64_t n = wbase + i * 64 + lane;
-        if (n < prm.n_out) out[n] = limb;
+        if (n < n_out) out[n] = limb;
         cin = cout;
     }
 }
 
 int64_t bw_carry_tiles(int64_t n_out) { return (n_out + BW_CARRY_TILE - 1) / BW_CARRY_TILE; }
+
+__global__ void carry_reset_kernel(unsigned long long* flags, int64_t ntiles, int* counter) {
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t i = i0; i < ntiles; i += (int64_t)gridDim.x * blockDim.x) flags[i] = 0ULL;
+    if (i0 == 0) *counter = 0;
+}
+
+hipError_t bw_launch_carry_reset(unsigned long long* flags, int64_t ntiles, int* counter, hipStream_t stream) {
+    int64_t blocks = (ntiles + 255) / 256;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(carry_reset_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, flags, ntiles, counter);
+    return hipGetLastError();
+}
 
 hipError_t bw_launch_garner_carry(const CarryParams& p, hipStream_t stream) {
     dim3 grid((unsigned)p.ntiles), block(256);

@@ -1,0 +1,224 @@
+// engine.cpp -- errors, table registry and the stacked transform sequence.
+#include "engine.h"
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+
+#include "../../include/birdwing.h"
+
+namespace bw {
+
+// ---------------------------------------------------------------------------
+// errors
+static thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+const char* last_error() { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------------
+// per (device, set) tables
+static std::mutex g_tab_mu;
+static std::map<std::pair<int, int>, std::unique_ptr<DevTables>> g_tabs;
+
+int get_tables(int set_id, DevTables** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    auto key = std::make_pair(dev, set_id);
+    auto it = g_tabs.find(key);
+    if (it != g_tabs.end()) { *out = it->second.get(); return BW_OK; }
+    const PrimeSet* s = prime_set(set_id);
+    if (!s) return fail(BW_ERR_ARG, "unknown prime set %d", set_id);
+    HostTables h;
+    build_tables(*s, h);
+    std::unique_ptr<DevTables> d(new DevTables());
+    d->t.limb_bits = s->limb_bits;
+    d->t.k = s->k;
+    d->t.table_log = h.table_log;
+    void** ptrs[4] = {&d->t.consts, &d->t.tw_stage, &d->t.tw_lo, &d->t.tw_hi};
+    std::vector<unsigned char>* srcs[4] = {&h.consts, &h.tw_stage, &h.tw_lo, &h.tw_hi};
+    for (int i = 0; i < 4; i++) {
+        HIP_TRY(hipMalloc(ptrs[i], srcs[i]->size()));
+        HIP_TRY(hipMemcpy(*ptrs[i], srcs[i]->data(), srcs[i]->size(), hipMemcpyHostToDevice));
+    }
+    d->garner = garner_constants(*s);
+    for (int i = 0; i < 3; i++) f64_prime(i < s->k ? s->p[i] : 3, d->fp[i]);
+    *out = d.get();
+    g_tabs[key] = std::move(d);
+    return BW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// pass geometry
+void default_passes(int logN, std::vector<int>& out) {
+    out.clear();
+    const int P = (logN + 8) / 9 < 1 ? 1 : (logN + 8) / 9;
+    const int base = logN / P, extra = logN % P;
+    for (int j = 0; j < P; j++) out.push_back(base + (j >= P - extra ? 1 : 0));
+}
+
+PassPlan pass_geom(const Xform& x, int j) {
+    PassPlan g;
+    int prefix = 0;
+    for (int i = 0; i < j; i++) prefix += x.passes[i];
+    g.logT = x.passes[j];
+    g.logM = x.logN - prefix;
+    g.S = (int64_t)1 << (x.logN - prefix - g.logT);
+    g.O = x.B << prefix;
+    int gmax = bw_pass_max_G(g.logT) >> x.gshift;
+    if (gmax < 1) gmax = 1;
+    if (g.S > 1) {
+        g.G = (int)(g.S < gmax ? g.S : gmax);
+    } else {
+        // rows: G * T <= 4096 keeps the fused kernel's per-thread registers small
+        while (gmax > 1 && ((int64_t)gmax << g.logT) > 4096) gmax >>= 1;
+        g.G = (int)(g.O < gmax ? g.O : gmax);
+        while (g.G > 1 && g.O % g.G) g.G >>= 1;     // whole tiles only
+    }
+    g.tiles = g.O * g.S / g.G;
+    return g;
+}
+
+PassParams base_params(const Xform& x, const PassPlan& g) {
+    PassParams prm;
+    std::memset(&prm, 0, sizeof(prm));
+    prm.N = x.N;
+    prm.plane = x.N * x.B;
+    prm.S = g.S;
+    prm.O = g.O;
+    prm.logM = g.logM;
+    prm.table_log = x.tabs->t.table_log;
+    prm.consts = x.tabs->t.consts;
+    prm.tw_stage = x.tabs->t.tw_stage;
+    prm.tw_lo = x.tabs->t.tw_lo;
+    prm.tw_hi = x.tabs->t.tw_hi;
+    prm.ninv = x.ninv;
+    prm.canon_out = 1;
+    return prm;
+}
+
+hipError_t launch_pass(const Xform& x, int mode, const PassPlan& g, const PassParams& prm, int nops,
+                       hipStream_t st) {
+    PassLaunch L;
+    L.mode = mode;
+    L.logT = g.logT;
+    L.G = g.G;
+    L.limb_bits = x.limb_bits;
+    L.nprimes = x.set->k;
+    L.nops = nops;
+    L.tiles = g.tiles;
+    return bw_launch_pass(L, prm, st);
+}
+
+int count_transform_launches(const Xform& x) {
+    const int P = (int)x.passes.size();
+    if (P == 1) return 1;
+    if (x.schedule == BW_SCHED_PRIME_MAJOR) return x.set->k * (2 * (P - 1) + 1 + (P - 1));
+    return (P - 1) + 1 + (P - 1);
+}
+
+#define MARK()                                  \
+    do {                                        \
+        if (mk) LAUNCH_TRY(mk->mark());         \
+    } while (0)
+
+// BW_SCHED_PASS_MAJOR: one launch per pass over every (prime, operand).
+// BW_SCHED_PRIME_MAJOR: the chain of one prime -- forward passes of A_p, of
+// B_p, the fused pass, the inverse passes -- runs before the next prime
+// starts, so each intermediate plane is re-read while the 256 MB Infinity
+// Cache still holds it.
+int run_transforms(const Xform& x, const void* d_a, const void* d_b, int64_t la, int64_t lb, hipStream_t st,
+                   LaunchMarker* mk) {
+    const int P = (int)x.passes.size();
+    if (P == 1) {
+        PassPlan g = pass_geom(x, 0);
+        PassParams prm = base_params(x, g);
+        prm.ingress = 1;
+        prm.in[0] = d_a;
+        prm.in[1] = d_b;
+        prm.in_len[0] = la;
+        prm.in_len[1] = lb;
+        prm.bdesc = x.bdesc;
+        prm.out[0] = x.bufA;
+        LAUNCH_TRY(launch_pass(x, BW_MODE_FUSED, g, prm, 1, st));
+        MARK();
+        return BW_OK;
+    }
+    // stacked ingress names the operand by launch slot (A in 0, B in 1)
+    if (x.bdesc && x.schedule == BW_SCHED_PRIME_MAJOR)
+        return fail(BW_ERR_ARG, "stacked ingress needs the pass-major schedule");
+    const bool folded = pass_geom(x, P - 1).logT <= BW_REG2_MAX_LOG && pass_geom(x, P - 2).logT <= BW_REG2_MAX_LOG;
+    const bool pm = x.schedule == BW_SCHED_PRIME_MAJOR;
+    const int k = x.set->k;
+    const int nchunk = pm ? k : 1;          // prime chunks
+    const int nper = pm ? 1 : k;            // primes per launch
+    const void* raw[2] = {d_a, d_b};
+    void* buf[2] = {x.bufA, x.bufB};
+    const int64_t len[2] = {la, lb};
+    for (int c = 0; c < nchunk; c++) {
+        const int prime0 = pm ? c : 0;
+        // forward passes 1 .. P-1: per operand (prime-major) or both operands at once
+        for (int op = 0; op < (pm ? 2 : 1); op++) {
+            for (int j = 0; j < P - 1; j++) {
+                PassPlan g = pass_geom(x, j);
+                PassParams prm = base_params(x, g);
+                prm.prime0 = prime0;
+                prm.canon_out = 0;
+                const int nops = pm ? 1 : 2;
+                for (int q = 0; q < nops; q++) {
+                    const int which = pm ? op : q;
+                    prm.in[q] = j == 0 ? raw[which] : (const void*)buf[which];
+                    prm.in_len[q] = len[which];
+                    prm.out[q] = buf[which];
+                }
+                if (j == 0) prm.bdesc = x.bdesc;
+                prm.ingress = j == 0;
+                prm.nops = nops;
+                PassLaunch L = {BW_MODE_FWD, g.logT, g.G, x.limb_bits, nper, nops, g.tiles};
+                LAUNCH_TRY(bw_launch_pass(L, prm, st));
+                MARK();
+            }
+        }
+        {
+            PassPlan g = pass_geom(x, P - 1);
+            PassParams prm = base_params(x, g);
+            prm.prime0 = prime0;
+            prm.in[0] = x.bufA;
+            prm.in[1] = x.bufB;
+            prm.out[0] = x.bufA;
+            prm.canon_out = 0;
+            prm.fold_ninv = folded;
+            PassLaunch L = {BW_MODE_FUSED, g.logT, g.G, x.limb_bits, nper, 1, g.tiles};
+            LAUNCH_TRY(bw_launch_pass(L, prm, st));
+            MARK();
+        }
+        for (int j = P - 2; j >= 0; j--) {
+            PassPlan g = pass_geom(x, j);
+            PassParams prm = base_params(x, g);
+            prm.prime0 = prime0;
+            prm.in[0] = x.bufA;
+            prm.out[0] = x.bufA;
+            prm.canon_out = j == 0 ? 1 : 0;
+            prm.fold_ninv = (j == P - 2) && folded;
+            PassLaunch L = {BW_MODE_INV, g.logT, g.G, x.limb_bits, nper, 1, g.tiles};
+            LAUNCH_TRY(bw_launch_pass(L, prm, st));
+            MARK();
+        }
+    }
+    return BW_OK;
+}
+
+}  // namespace bw

@@ -73,10 +73,11 @@ struct ArF64x {
         const double s = step.x, sp = step.y;
         const double s2 = f64_mulmod(s, s, sp, P), s2p = s2 * P.pinv;
         const double s4 = f64_mulmod(s2, s2, s2p, P), s4p = s4 * P.pinv;
+        // w holds NTW entries: the first four are written only where they exist
         w[0] = base.x;
-        w[1] = f64_mulmod(base.x, s, sp, P);
-        w[2] = f64_mulmod(base.x, s2, s2p, P);
-        w[3] = f64_mulmod(w[1], s2, s2p, P);
+        if constexpr (NTW > 1) w[1] = f64_mulmod(base.x, s, sp, P);
+        if constexpr (NTW > 2) w[2] = f64_mulmod(base.x, s2, s2p, P);
+        if constexpr (NTW > 3) w[3] = f64_mulmod(w[1], s2, s2p, P);
 #pragma unroll
         for (int k = 4; k < NTW; k++) w[k] = f64_mulmod(w[k - 4], s4, s4p, P);
     }
@@ -137,9 +138,9 @@ struct ArU64x {
         const uint64_t s = u64_montmul(step.x, P.r2, P);
         const uint64_t s2 = u64_montmul(s, s, P), s4 = u64_montmul(s2, s2, P);
         w[0] = u64_montmul(base.x, P.r2, P);
-        w[1] = u64_montmul(w[0], s, P);
-        w[2] = u64_montmul(w[0], s2, P);
-        w[3] = u64_montmul(w[1], s2, P);
+        if constexpr (NTW > 1) w[1] = u64_montmul(w[0], s, P);
+        if constexpr (NTW > 2) w[2] = u64_montmul(w[0], s2, P);
+        if constexpr (NTW > 3) w[3] = u64_montmul(w[1], s2, P);
 #pragma unroll
         for (int k = 4; k < NTW; k++) w[k] = u64_montmul(w[k - 4], s4, P);
     }
@@ -250,14 +251,28 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     const int g0 = COLS ? (tid & (G - 1)) : (tid >> R1LOG);
     const int j0 = COLS ? (tid >> logG) : (tid & (R1 - 1));
 
+    // raw limbs of operand `which` for this thread's line; `base` becomes the
+    // position inside the product (stacked products: line o reads product o)
+    auto ingress_src = [&](int which, const typename AR::Raw*& raw, int64_t& len, int64_t& base) {
+        raw = reinterpret_cast<const typename AR::Raw*>(prm.in[which]);
+        len = prm.in_len[which];
+        if (prm.bdesc) {
+            const int64_t o = COLS ? o0 : o0 + g0;
+            const BwIngress d = prm.bdesc[o];
+            raw += which ? d.b_off : d.a_off;
+            len = which ? d.b_len : d.a_len;
+            base -= o << prm.logM;
+        }
+    };
     // stage-0 input words of one operand (raw limbs or residue words)
     using IW = typename std::conditional<INGRESS, typename AR::Raw, W>::type;
     auto load0 = [&](int which, IW* w) {
-        const int64_t base = line_base(g0) + j0 * tstride;
+        int64_t base = line_base(g0) + j0 * tstride;
         const int64_t step = (int64_t)R1 * tstride;
         if constexpr (INGRESS) {
-            const typename AR::Raw* raw = reinterpret_cast<const typename AR::Raw*>(prm.in[which]);
-            const int64_t len = prm.in_len[which];
+            const typename AR::Raw* raw;
+            int64_t len;
+            ingress_src(which, raw, len, base);
 #pragma unroll
             for (int m = 0; m < R0; m++) {
                 const int64_t a = base + m * step;
@@ -283,12 +298,13 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         for (int k = 0; k < R0; k++) lds[g0 * LT + T2Lds<LOGT>::pos(k * R1 + j0)] = AR::to_lds(x[brev(k, R0LOG)], P);
     };
     auto fwd_stage0 = [&](int which) {
-        const int64_t base = line_base(g0) + j0 * tstride;
+        int64_t base = line_base(g0) + j0 * tstride;
         const int64_t step = (int64_t)R1 * tstride;
         V x[R0];
         if constexpr (INGRESS) {
-            const typename AR::Raw* raw = reinterpret_cast<const typename AR::Raw*>(prm.in[which]);
-            const int64_t len = prm.in_len[which];
+            const typename AR::Raw* raw;
+            int64_t len;
+            ingress_src(which, raw, len, base);
 #pragma unroll
             for (int m = 0; m < R0; m++) {
                 const int64_t a = base + m * step;

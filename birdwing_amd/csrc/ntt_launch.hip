@@ -1,6 +1,9 @@
 // ntt_launch.hip -- kernel selection and launch of one NTT pass.
 #include "ntt_kernels.h"
 
+#include <mutex>
+#include <unordered_set>
+
 typedef void (*PassKernel)(PassParams);
 PassKernel bw_pick_f64_fwd(int logT, int G);
 PassKernel bw_pick_f64_inv(int logT, int G);
@@ -56,13 +59,21 @@ hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_
     // LDS element: 8 bytes, except the register-staged 32-bit path (centered int32)
     const size_t vbytes = (reg2 && L.limb_bits == 32) ? 4 : 8;
     const size_t lds = (size_t)L.G * (reg2 ? bw_reg2_lds_line(L.logT) : lds_line(L.logT)) * vbytes;
-    static bool attr_set[2][3][16][33][4] = {};
-    bool& done = attr_set[L.limb_bits == 64][L.mode][L.logT][L.G][(prm.S > 1) * 2 + (prm.ingress != 0)];
-    if (!done) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        done = true;
+    if (L.G > bw_pass_max_G(L.logT) || (L.G & (L.G - 1))) return hipErrorInvalidValue;
+    if (prm.bdesc && prm.ingress && !reg2) return hipErrorInvalidValue;   // stacked ingress: reg2 only
+    {
+        // Allow the largest tile this kernel is ever launched with, once per kernel.
+        static std::mutex mu;
+        static std::unordered_set<const void*> done;
+        std::lock_guard<std::mutex> lock(mu);
+        if (!done.count(reinterpret_cast<const void*>(k))) {
+            const size_t lds_max = (size_t)bw_pass_max_G(L.logT) *
+                                   (reg2 ? bw_reg2_lds_line(L.logT) : lds_line(L.logT)) * vbytes;
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max);
+            if (e != hipSuccess) return e;
+            done.insert(reinterpret_cast<const void*>(k));
+        }
     }
     dim3 grid((unsigned)L.tiles, (unsigned)L.nprimes, (unsigned)(L.mode == BW_MODE_FWD ? L.nops : 1));
     // register-staged kernels run G * R1 threads (one stage-0 group each)

@@ -157,6 +157,50 @@ int bw_carry128(const uint64_t* d_hi, const uint64_t* d_lo, uint64_t N, void* d_
 int bw_carry160(const uint64_t* d_lo, const uint64_t* d_mid, const uint32_t* d_hi, uint64_t N, void* d_out,
                 uint64_t n_out, void* stream);
 
+/* ---- batched products (no counterpart in the reference, whose API runs one
+ *      product per call: execute_ntt_multiply, src/gpu_ntt.cu:442-973) --------
+ * n products of uneven sizes; limb_bits[i] is 32 or 64 per product.  Operands
+ * and results are packed: product i reads L_A[i] limbs at d_a + off_a[i] and
+ * L_B[i] limbs at d_b + off_b[i] (byte offsets from bw_batch_offsets, 256-byte
+ * aligned) and writes L_A[i] + L_B[i] limbs at d_c + off_c[i].  Products with
+ * the same limb width and padded size share every launch; one Garner+carry
+ * launch per prime set covers the whole batch.  The batch owns its device
+ * workspace (bw_batch_info.workspace_bytes). */
+typedef struct bw_batch bw_batch;
+typedef struct bw_batch_info {
+    int n;                   /* products */
+    int ngroups;             /* (limb width, padded size) groups */
+    int nlaunch;             /* kernel launches per bw_batch_execute */
+    int use_graph;
+    uint64_t bytes_a, bytes_b, bytes_c;   /* packed buffer sizes */
+    uint64_t workspace_bytes;
+} bw_batch_info;
+int bw_batch_create(bw_batch** batch, int n, const int* limb_bits, const uint64_t* L_A, const uint64_t* L_B);
+void bw_batch_destroy(bw_batch* batch);
+/* Host-only: device workspace and packed buffer bytes a batch of these sizes
+ * needs (admission control; no device is touched). */
+int bw_batch_footprint(int n, const int* limb_bits, const uint64_t* L_A, const uint64_t* L_B,
+                       uint64_t* workspace_bytes, uint64_t* bytes_a, uint64_t* bytes_b, uint64_t* bytes_c);
+int bw_batch_get_info(const bw_batch* batch, bw_batch_info* info);
+int bw_batch_offsets(const bw_batch* batch, uint64_t* off_a, uint64_t* off_b, uint64_t* off_c);
+/* Group gi: limb width, log2 N, products, stacked slots, pass split, tile cap. */
+int bw_batch_group_info(const bw_batch* batch, int gi, int* limb_bits, int* logN, int* members, int* slots,
+                        int* pass_log, int* npass, int* gshift);
+/* Runs the batch on `stream` with no host synchronization. */
+int bw_batch_execute(bw_batch* batch, const void* d_a, const void* d_b, void* d_c, void* stream);
+/* enabled: capture the launch sequence into a hipGraph on the first execute on
+ * a non-null stream and replay it while (d_a, d_b, d_c, stream) stay the same. */
+int bw_batch_set_graph(bw_batch* batch, int enabled);
+/* Autotuner: times candidate pass splits and tile caps of every group on the
+ * given operands (reps timed runs each, synchronizing), keeps the fastest in
+ * the batch and in a process-wide table that later batches of the same
+ * (limb width, size, group-size bucket) start from. */
+int bw_batch_tune(bw_batch* batch, const void* d_a, const void* d_b, void* stream, int reps);
+int bw_tune_count(void);
+void bw_tune_clear(void);
+/* Synchronizes; BW_ERR_STATE if a carry look-back wait exceeded its bound. */
+int bw_batch_status(bw_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

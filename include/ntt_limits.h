@@ -1,0 +1,88 @@
+// ntt_limits.h -- drop-in for the reference's include/ntt_limits.h
+// (src/ntt_limits.cpp), answered by libbirdwing_hip.so's size policy.
+//
+// The configured limits are this library's: for 32-bit limbs the reference
+// prime set serves N <= 2^23 and the wide set {2013265921, 1811939329,
+// 469762049} extends it to N = 2^26 (so max_supported_logN() == 26 where the
+// reference reports 23); for 64-bit limbs the reference 58-bit set (N <= 2^32).
+// The per-prime helpers are exact restatements.  ensure_*() print to stderr
+// and exit(1), as in the reference.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+
+#include "birdwing.h"
+#include "config.h"
+
+namespace bw_dropin {
+inline int widest_set() { return BW_CFG_LIMB_BITS == 64 ? BW_SET_U64_REF : BW_SET_U32_WIDE; }
+inline int set_for(size_t N) { return bw_default_prime_set(BW_CFG_LIMB_BITS, N); }
+}  // namespace bw_dropin
+
+inline int max_logN_for_prime(uint64_t p) {
+    int k = 0;
+    uint64_t n = p - 1;
+    while (n && !(n & 1)) { n >>= 1; k++; }
+    return k;
+}
+
+inline int max_root_logN(TestDataTypeUint root, TestDataTypeUint p) {
+    unsigned __int128 r = root % p;
+    if (r == 0) return 0;
+    for (int k = 0; k < 62; k++) {
+        if (r == 1) return k;
+        r = r * r % p;
+    }
+    return 62;
+}
+
+inline int max_supported_logN() { return bw_max_supported_logN(bw_dropin::widest_set()); }
+inline size_t max_supported_N() { return size_t(1) << max_supported_logN(); }
+inline size_t max_supported_limb_count() {
+    return (size_t)bw_max_supported_limb_count(bw_dropin::widest_set(), INPUT_LIMB_BITS);
+}
+inline size_t padded_ntt_size(size_t L_A, size_t L_B) { return (size_t)bw_padded_ntt_size(L_A, L_B); }
+
+inline bool ntt_size_supported(size_t N, std::string* why = nullptr) {
+    char buf[256] = {0};
+    const int ok = (N && !(N & (N - 1))) ? bw_ntt_size_supported(bw_dropin::set_for(N), N, buf, sizeof buf)
+                                         : bw_ntt_size_supported(BW_SET_U32_REF, N, buf, sizeof buf);
+    if (!ok && why) *why = buf;
+    return ok != 0;
+}
+
+inline bool crt_coefficient_bound_satisfied(size_t L_A, size_t L_B, std::string* why = nullptr) {
+    char buf[256] = {0};
+    const int ok = bw_crt_bound_satisfied(bw_dropin::set_for(padded_ntt_size(L_A ? L_A : 1, L_B ? L_B : 1)),
+                                          INPUT_LIMB_BITS, L_A, L_B, buf, sizeof buf);
+    if (!ok && why) *why = buf;
+    return ok != 0;
+}
+
+inline bool multiply_size_supported(size_t L_A, size_t L_B, std::string* why = nullptr) {
+    char buf[256] = {0};
+    const int ok = bw_multiply_size_supported(-1, INPUT_LIMB_BITS, L_A, L_B, buf, sizeof buf);
+    if (!ok && why) *why = buf;
+    return ok != 0;
+}
+
+inline void ensure_ntt_size_supported(size_t N) {
+    std::string why;
+    if (!ntt_size_supported(N, &why)) {
+        std::fprintf(stderr, "[NTT] Unsupported size: %s\n[NTT] Max supported N=2^%d (%zu)\n", why.c_str(),
+                     max_supported_logN(), max_supported_N());
+        std::exit(1);
+    }
+}
+
+inline void ensure_multiply_size_supported(size_t L_A, size_t L_B) {
+    std::string why;
+    if (!multiply_size_supported(L_A, L_B, &why)) {
+        std::fprintf(stderr, "[NTT] Unsupported multiply: %s\n[NTT] Max supported N=2^%d (%zu), max L=%zu\n",
+                     why.c_str(), max_supported_logN(), max_supported_N(), max_supported_limb_count());
+        std::exit(1);
+    }
+}

@@ -314,7 +314,7 @@ def test_product_carry_stress(gpu, oracle, bits):
     _product_check(gpu, oracle, A, A, bits)
 
 
-@pytest.mark.parametrize("passes", [[1, 12], [6, 7], [7, 6], [12, 1], [4, 4, 5], [2, 3, 8]])
+@pytest.mark.parametrize("passes", [[1, 12], [6, 7], [7, 6], [12, 1], [4, 4, 5], [2, 3, 8], [1, 1, 11], [11, 1, 1], [3, 9, 1]])
 def test_product_pass_splits(gpu, oracle, passes):
     """Every pass decomposition gives the same product (N = 2^13)."""
     A = oracle.uniform_limbs(4096, 11)
