@@ -140,6 +140,284 @@ def cpu_baselines(A, B, bits, gpu_out, sample_oracle):
     return res, exact
 
 
+def product_model_bytes(LA, LB, bits, k=3):
+    """Staged-model bytes of one product (32-bit: 136 B per coefficient + limbs)."""
+    import birdwing_amd as bw
+    w = bits // 8
+    wc = 16 if bits == 32 else 20
+    N = bw.padded_ntt_size(int(LA), int(LB))
+    return (int(LA) + int(LB)) * 2 * w + 8 * k * N * w + 2 * N * wc
+
+
+def bench_batch(n, steps, warmup, seed=4096, host_sample=128):
+    """Batched workload: n products, limb counts per operand log-uniform in
+    [2^8, 2^16] (seeded), uniform random 32-bit limbs (top limb nonzero);
+    autotuned, hipGraph-captured; against the plain per-product path on the
+    same products (device-resident Plan.execute, plans made beforehand) and
+    the host one-shot path (plan, copies, execute, release) on a sample."""
+    import torch
+    import birdwing_amd as bw
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    rng = np.random.default_rng(seed)
+    LA = np.floor(2.0 ** rng.uniform(8, 16, n)).astype(np.int64)
+    LB = np.floor(2.0 ** rng.uniform(8, 16, n)).astype(np.int64)
+    b = bw.Batch(32, LA, LB)
+    pa = rng.integers(0, 2**32, b.bytes_a // 4, dtype=np.uint64).astype(np.uint32)
+    pb = rng.integers(0, 2**32, b.bytes_b // 4, dtype=np.uint64).astype(np.uint32)
+    for i in range(n):                  # top limbs nonzero
+        pa[(b.off_a[i] >> 2) + LA[i] - 1] |= 1
+        pb[(b.off_b[i] >> 2) + LB[i] - 1] |= 1
+    da = torch.from_numpy(pa.view(np.uint8)).cuda()
+    db = torch.from_numpy(pb.view(np.uint8)).cuda()
+    dc = torch.zeros(b.bytes_c, dtype=torch.uint8, device="cuda")
+    s = torch.cuda.Stream()
+    t_tune = time.perf_counter()
+    with torch.cuda.stream(s):
+        b.tune(da, db, stream=s, reps=2)
+    t_tune = time.perf_counter() - t_tune
+    b.set_graph(True)
+    with torch.cuda.stream(s):
+        for _ in range(warmup):
+            b.execute(da, db, dc, stream=s)
+        s.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        t0 = time.perf_counter()
+        ev[0].record(s)
+        for i in range(steps):
+            b.execute(da, db, dc, stream=s)
+            ev[i + 1].record(s)
+        s.synchronize()
+        t1 = time.perf_counter()
+    b.status()
+    lat = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(steps))
+    batch_s = (t1 - t0) / steps
+    out = dc.cpu().numpy()
+    got = b.unpack(out)
+    mbytes = sum(product_model_bytes(x, y, 32) for x, y in zip(LA, LB))
+
+    # plain per-product path on the same products, same device buffers
+    plans = [bw.Plan(32, int(LA[i]), int(LB[i])) for i in range(n)]
+    av = [da[b.off_a[i]:b.off_a[i] + 4 * int(LA[i])] for i in range(n)]
+    bv = [db[b.off_b[i]:b.off_b[i] + 4 * int(LB[i])] for i in range(n)]
+    pc = torch.zeros(2 * (1 << 16) + 8, dtype=torch.int32, device="cuda")
+    with torch.cuda.stream(s):
+        for i in range(min(n, 64)):
+            plans[i].execute(av[i], bv[i], pc, stream=s)
+        s.synchronize()
+        same = True
+        t2 = time.perf_counter()
+        for i in range(n):
+            plans[i].execute(av[i], bv[i], pc, stream=s)
+        s.synchronize()
+        t3 = time.perf_counter()
+        for i in range(0, n, max(1, n // 64)):      # the plain path agrees with the batch
+            plans[i].execute(av[i], bv[i], pc, stream=s)
+            s.synchronize()
+            m = int(LA[i] + LB[i])
+            same = same and bool(np.array_equal(pc[:m].cpu().numpy().view(np.uint32), got[i]))
+    plain_s = (t3 - t2) / n
+    del plans
+    # host one-shot path (the reference's host_multiply_merge shape) on a sample
+    idx = list(range(0, n, max(1, n // host_sample)))[:host_sample]
+    t4 = time.perf_counter()
+    exact = True
+    import pyoracle
+    for j, i in enumerate(idx):
+        A = pa[(b.off_a[i] >> 2):(b.off_a[i] >> 2) + LA[i]]
+        B = pb[(b.off_b[i] >> 2):(b.off_b[i] >> 2) + LB[i]]
+        r = bw.multiply(A, B, 32)
+        if j < 16 and pyoracle.gmp_available():
+            g, _ = pyoracle.gmp_mul(A, B, 32)
+            exact = exact and bool(np.array_equal(g, r)) and bool(np.array_equal(g, got[i]))
+    host_s = (time.perf_counter() - t4) / len(idx)
+    groups = b.groups()
+    return {
+        "workload": "batch_%d_logu_2^8-2^16_u32" % n,
+        "products": n,
+        "steps": steps,
+        "products_per_s": n / batch_s,
+        "ms_per_batch": batch_s * 1e3,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "alg_bytes": mbytes,
+        "achieved": mbytes / batch_s / 1e9,
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": mbytes / batch_s / 1e9 / HBM_PEAK_GBPS,
+        "plain_device_products_per_s": 1.0 / plain_s,
+        "plain_host_products_per_s": 1.0 / host_s,
+        "vs_plain_device": plain_s / batch_s * n,
+        "vs_plain_host": host_s / batch_s * n,
+        "bit_exact_vs_gmp_sample": exact,
+        "same_as_plain_sample": same,
+        "graph": True,
+        "launches": int(b.info.nlaunch),
+        "autotune_s": t_tune,
+        "groups": [{"logN": g["logN"], "members": g["members"], "passes": g["passes"], "gshift": g["gshift"]}
+                   for g in groups],
+    }
+
+
+class _ServeMix:
+    """Serving workload: limb counts log-uniform in [2^6, 2^22] per operand,
+    80% 32-bit / 20% 64-bit digits, uniform random limbs taken as slices of
+    pinned random pools; results land in a pinned ring (sampled requests get
+    their own arrays for checking)."""
+
+    def __init__(self, seed, ring_bytes=1 << 30):
+        import torch
+        self.rng = np.random.default_rng(seed)
+        words = 1 << 23
+        self.pool = {}
+        for bits, dt, tdt in ((32, np.uint32, torch.int32), (64, np.uint64, torch.int64)):
+            pa = torch.empty(words, dtype=tdt, pin_memory=True).numpy().view(dt)
+            pb = torch.empty(words, dtype=tdt, pin_memory=True).numpy().view(dt)
+            pa[:] = self.rng.integers(0, 2**bits, words, dtype=np.uint64, endpoint=False).astype(dt)
+            pb[:] = self.rng.integers(0, 2**bits, words, dtype=np.uint64, endpoint=False).astype(dt)
+            pa[pa == 0] = 1
+            pb[pb == 0] = 1
+            self.pool[bits] = (pa, pb)
+        self.ring = torch.empty(ring_bytes, dtype=torch.uint8, pin_memory=True).numpy()
+        self.cur = 0
+
+    def next(self):
+        bits = 64 if self.rng.random() < 0.2 else 32
+        la = int(2.0 ** self.rng.uniform(6, 22))
+        lb = int(2.0 ** self.rng.uniform(6, 22))
+        pa, pb = self.pool[bits]
+        oa = int(self.rng.integers(0, len(pa) - la))
+        ob = int(self.rng.integers(0, len(pb) - lb))
+        return bits, pa[oa:oa + la], pb[ob:ob + lb]
+
+    def out(self, n, bits):
+        nb = n * bits // 8
+        if self.cur + nb > len(self.ring):
+            self.cur = 0
+        v = self.ring[self.cur:self.cur + nb].view(np.uint64 if bits == 64 else np.uint32)
+        self.cur = (self.cur + nb + 255) // 256 * 256
+        return v
+
+
+def bench_serve(seconds, seed=7, sat_seconds=10.0, plain_seconds=10.0):
+    """Poisson request stream through the server at 80% of the plain path's
+    sustained rate, a saturated run, and the plain per-request path (host
+    one-shot multiply: plan, copies, execute, release per request)."""
+    import threading
+    import birdwing_amd as bw
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    mix = _ServeMix(seed)
+    # plain per-request path, closed loop
+    lat_plain = []
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < plain_seconds:
+        bits, A, B = mix.next()
+        t = time.perf_counter()
+        bw.multiply(A, B, bits)
+        lat_plain.append((time.perf_counter() - t) * 1e3)
+    plain_rate = len(lat_plain) / (time.perf_counter() - t0)
+
+    def run(rate, duration, max_outstanding=None, check_every=0):
+        srv = bw.Server(device_bytes=32 << 30, streams=3, max_batch=1024)
+        tickets = []
+        checks = []
+        lock = threading.Lock()
+        done = threading.Event()
+        lats = []
+
+        def producer():
+            t_start = time.perf_counter()
+            nxt = t_start
+            i = 0
+            rng = np.random.default_rng(seed + 1)
+            while True:
+                now = time.perf_counter()
+                if now - t_start >= duration:
+                    break
+                if rate is not None:
+                    if now < nxt:
+                        time.sleep(min(nxt - now, 0.001))
+                        continue
+                    nxt += rng.exponential(1.0 / rate)
+                elif max_outstanding is not None:
+                    with lock:
+                        outstanding = len(tickets) - len(lats)
+                    if outstanding >= max_outstanding:
+                        time.sleep(0.0002)
+                        continue
+                bits, A, B = mix.next()
+                if check_every and i % check_every == 0:
+                    out = np.zeros(len(A) + len(B), dtype=A.dtype)
+                    checks.append((bits, A.copy(), B.copy(), out))
+                else:
+                    out = mix.out(len(A) + len(B), bits)
+                t = srv.submit(A, B, out, bits)
+                with lock:
+                    tickets.append(t)
+                i += 1
+            done.set()
+
+        def waiter():
+            j = 0
+            while True:
+                with lock:
+                    have = j < len(tickets)
+                    t = tickets[j] if have else None
+                if not have:
+                    if done.is_set():
+                        with lock:
+                            if j >= len(tickets):
+                                return
+                    time.sleep(0.0001)
+                    continue
+                lat = srv.wait(t)
+                with lock:
+                    lats.append(lat)
+                j += 1
+
+        th_p = threading.Thread(target=producer)
+        th_w = threading.Thread(target=waiter)
+        t_begin = time.perf_counter()
+        th_p.start()
+        th_w.start()
+        th_p.join()
+        th_w.join()
+        elapsed = time.perf_counter() - t_begin
+        st = srv.stats()
+        srv.close()
+        return len(lats) / elapsed, lats, st, checks
+
+    served_rate, lat_s, st_s, checks = run(0.8 * plain_rate, seconds, check_every=50)
+    sat_rate, lat_sat, st_sat, _ = run(None, sat_seconds, max_outstanding=256)
+    # sampled results of the Poisson run equal the same products alone
+    import pyoracle
+    exact = True
+    for bits, A, B, out in checks:
+        alone = bw.multiply(A, B, bits)
+        exact = exact and bool(np.array_equal(alone, out))
+        if len(A) + len(B) <= (1 << 17) and pyoracle.gmp_available():
+            g, _ = pyoracle.gmp_mul(A, B, bits)
+            exact = exact and bool(np.array_equal(g, out))
+    pct = lambda v, q: float(np.percentile(v, q)) if v else None
+    return {
+        "workload": "serve_poisson_logu_2^6-2^22_80u32_20u64",
+        "seconds": seconds,
+        "arrival_rate": 0.8 * plain_rate,
+        "products_per_s": served_rate,
+        "latency_ms": {"p50": pct(lat_s, 50), "p99": pct(lat_s, 99)},
+        "requests": len(lat_s),
+        "batches": int(st_s["batches"]),
+        "mean_batch": st_s["batched_products"] / max(1, st_s["batches"]),
+        "max_concurrent_batches": int(st_s["max_concurrent_batches"]),
+        "saturated_products_per_s": sat_rate,
+        "saturated_latency_ms": {"p50": pct(lat_sat, 50), "p99": pct(lat_sat, 99)},
+        "saturated_mean_batch": st_sat["batched_products"] / max(1, st_sat["batches"]),
+        "plain_products_per_s": plain_rate,
+        "plain_latency_ms": {"p50": pct(lat_plain, 50), "p99": pct(lat_plain, 99)},
+        "saturated_vs_plain": sat_rate / plain_rate,
+        "same_as_alone_sample": exact,
+        "checked": len(checks),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,9 +426,12 @@ def main():
     ap.add_argument("--limbs", type=int, default=1 << 24, help="limbs per operand")
     ap.add_argument("--bits", type=int, default=32, choices=(32, 64))
     ap.add_argument("--passes", type=str, default="", help="pass split, e.g. 12,13")
-    ap.add_argument("--schedule", type=int, default=1, help="0 pass-major, 1 prime-major launches")
+    ap.add_argument("--schedule", type=int, default=0, help="0 pass-major (default), 1 prime-major launches")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-oracle", action="store_true", help="GMP only as CPU baseline")
+    ap.add_argument("--batch-n", type=int, default=4096, help="products in the batched workload (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="Poisson serving run length (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (--gpus 1); multi-GPU runs are not part of it",
@@ -253,6 +534,13 @@ def main():
         line["cpu_baseline"] = cpu["gmp"]
     if "gmp" in cpu:
         line["gmp_baseline"] = cpu["gmp"]
+    del plan, da, db, dc
+    torch.cuda.empty_cache()
+    if args.batch_n > 0:
+        line["batched"] = bench_batch(args.batch_n, args.batch_steps, 2)
+        torch.cuda.empty_cache()
+    if args.serve_seconds > 0:
+        line["served"] = bench_serve(args.serve_seconds)
     print(json.dumps(line))
 
 

@@ -26,7 +26,7 @@ __all__ = [
     "padded_ntt_size", "multiply_size_supported", "ntt_size_supported", "crt_bound_satisfied",
     "max_supported_logN", "max_supported_limb_count", "max_limb_count_by_crt", "default_prime_set",
     "prime_set_info", "crt_combine", "carry128", "carry160", "zero_pad", "zero_pad_u64", "pointwise",
-    "Batch", "batch_footprint", "tune_count", "tune_clear",
+    "Batch", "batch_footprint", "tune_count", "tune_clear", "Server",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -53,6 +53,16 @@ class BatchInfo(C.Structure):
     _fields_ = [("n", C.c_int), ("ngroups", C.c_int), ("nlaunch", C.c_int), ("use_graph", C.c_int),
                 ("bytes_a", C.c_uint64), ("bytes_b", C.c_uint64), ("bytes_c", C.c_uint64),
                 ("workspace_bytes", C.c_uint64)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("device_bytes", C.c_uint64), ("streams", C.c_int), ("max_batch", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("failed", C.c_int64), ("batches", C.c_int64),
+                ("batched_products", C.c_int64), ("pending", C.c_int64), ("max_concurrent_batches", C.c_int),
+                ("arena_bytes", C.c_uint64), ("arena_peak_bytes", C.c_uint64)]
 
 
 _vp, _u64, _i, _sz, _cp = C.c_void_p, C.c_uint64, C.c_int, C.c_size_t, C.c_char_p
@@ -104,6 +114,12 @@ SIGNATURES = {
     "bw_tune_count": (_i, []),
     "bw_tune_clear": (None, []),
     "bw_batch_status": (_i, [_vp]),
+    "bw_server_create": (_i, [C.POINTER(_vp), C.POINTER(ServerConfig)]),
+    "bw_server_submit": (C.c_int64, [_vp, _i, _vp, _u64, _vp, _u64, _vp]),
+    "bw_server_wait": (_i, [_vp, C.c_int64, _i, C.POINTER(C.c_double)]),
+    "bw_server_poll": (_i, [_vp, C.c_int64]),
+    "bw_server_get_stats": (_i, [_vp, C.POINTER(ServerStats)]),
+    "bw_server_destroy": (None, [_vp]),
 }
 
 _LIB = None
@@ -420,3 +436,59 @@ def tune_count():
 
 def tune_clear():
     lib().bw_tune_clear()
+
+
+# ---- serving -------------------------------------------------------------------
+class Server:
+    """Request loop with continuous batching on the current HIP device.
+
+    submit() takes host limb arrays (numpy; pinned memory lets the copies
+    overlap) and the result array (L_A + L_B limbs), which must stay alive
+    until wait() returns for the ticket; the server keeps references to them.
+    """
+
+    def __init__(self, device_bytes=8 << 30, streams=3, max_batch=1024):
+        cfg = ServerConfig(device_bytes, streams, max_batch)
+        self._s = C.c_void_p()
+        _check(lib().bw_server_create(C.byref(self._s), C.byref(cfg)))
+        self._keep = {}
+
+    def submit(self, A, B, out, limb_bits=None):
+        if limb_bits is None:
+            limb_bits = 64 if A.dtype == np.uint64 else 32
+        t = int(lib().bw_server_submit(self._s, limb_bits, _ptr(A), len(A), _ptr(B), len(B), _ptr(out)))
+        if t < 0:
+            _check(t)
+        self._keep[t] = (A, B, out)
+        return t
+
+    def wait(self, ticket, timeout_ms=-1):
+        """Blocks until the ticket completes; returns its latency in ms."""
+        lat = C.c_double()
+        rc = lib().bw_server_wait(self._s, ticket, timeout_ms, C.byref(lat))
+        self._keep.pop(ticket, None)
+        _check(rc)
+        return lat.value
+
+    def poll(self, ticket):
+        rc = int(lib().bw_server_poll(self._s, ticket))
+        if rc < 0:
+            _check(rc)
+        return rc == 1
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().bw_server_get_stats(self._s, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self._s:
+            lib().bw_server_destroy(self._s)
+            self._s = C.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -130,7 +130,7 @@ struct bw_plan {
     void* h_b = nullptr;
     void* h_c = nullptr;
     hipStream_t own = nullptr;
-    int schedule = BW_SCHED_PRIME_MAJOR;
+    int schedule = BW_SCHED_PASS_MAJOR;
     // timing: one event set (launches + 1 events) per recorded execution
     bool timing = false;
     std::vector<std::vector<hipEvent_t>> evpool;

@@ -111,6 +111,7 @@ int batch_layout(int n, const int* bits, const uint64_t* LA, const uint64_t* LB,
             g.gshift = 0;
         }
         group_geometry(g);
+        g.cap = g.B;
     }
     // carry launches: one per prime set
     std::map<int, int> cidx;
@@ -132,7 +133,7 @@ int batch_layout(int n, const int* bits, const uint64_t* LA, const uint64_t* LB,
     size_t off = 0;
     for (auto& g : L.groups) {
         const size_t w = (size_t)g.limb_bits / 8;
-        const size_t bytes = (size_t)prime_set(g.set_id)->k * (size_t)g.B * (size_t)g.N * w;
+        const size_t bytes = (size_t)prime_set(g.set_id)->k * (size_t)g.cap * (size_t)g.N * w;
         g.off_bufA = off;
         off = align_up(off + bytes, 256);
         g.off_bufB = off;
@@ -143,7 +144,7 @@ int batch_layout(int n, const int* bits, const uint64_t* LA, const uint64_t* LB,
         g.off_ninv = off;
         off = align_up(off + 16 * 3, 64);
         g.off_ing = off;
-        off = align_up(off + sizeof(BwIngress) * (size_t)g.B, 64);
+        off = align_up(off + sizeof(BwIngress) * (size_t)g.cap, 64);
     }
     for (auto& c : L.carries) {
         c.off_jobs = off;
@@ -492,7 +493,7 @@ extern "C" int bw_batch_tune(bw_batch* b, const void* d_a, const void* d_b, void
             t.passes = c.passes;
             t.gshift = c.gshift;
             group_geometry(t);
-            if (t.B != g.B) continue;            // slot count fixed by the workspace
+            if (t.B > g.cap) continue;           // more slots than the workspace holds
             float ms_min = -1.0f;
             for (int r = 0; r < reps + 1 && !rc; r++) {
                 if ((rc = (hipEventRecord(e0, st) == hipSuccess ? BW_OK : fail(BW_ERR_HIP, "hipEventRecord"))))
@@ -516,12 +517,18 @@ extern "C" int bw_batch_tune(bw_batch* b, const void* d_a, const void* d_b, void
         if (best.ms >= 0) {
             g.passes = best.passes;
             g.gshift = best.gshift;
+            group_geometry(g);
             tune_store(TuneKey{g.limb_bits, g.logN, log2_bucket(m)}, best);
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     drop_graph(b);
+    // slot counts may have changed: descriptors follow the tuned geometry
+    if (!rc) {
+        HIP_TRY(hipStreamSynchronize(st));
+        rc = upload_descriptors(b);
+    }
     return rc;
 }
 

@@ -22,6 +22,7 @@ struct BatchGroup {
     int logN = 0;
     int64_t N = 0;
     int64_t B = 0;                  // stacked slots (members padded to the tile multiple)
+    int64_t cap = 0;                // slots the workspace holds (B <= cap)
     std::vector<int> members;       // product indices, slot order
     std::vector<int> passes;
     int gshift = 0;

@@ -39,7 +39,7 @@ struct Xform {
     void* bufA = nullptr;
     void* bufB = nullptr;
     const void* ninv = nullptr;     // per-prime N^-1 pairs for this N
-    int schedule = 1;               // BW_SCHED_*
+    int schedule = 0;               // BW_SCHED_*
     const BwIngress* bdesc = nullptr;   // stacked ingress (B > 1), device memory
 };
 
@@ -68,7 +68,9 @@ int run_transforms(const Xform& x, const void* d_a, const void* d_b, int64_t la,
                    LaunchMarker* mk);
 int count_transform_launches(const Xform& x);
 
-// Balanced split of 2^logN into passes of at most 2^9 points, largest last.
+// Default split of 2^logN into passes: as balanced as possible with at most
+// 2^9 points each, except that the last (fused) pass keeps at most 2^8 points
+// and the first (ingress) pass is not enlarged while a middle pass can be.
 void default_passes(int logN, std::vector<int>& out);
 
 }  // namespace bw

@@ -197,6 +197,11 @@ struct T2Lds {
 };
 
 // w_M^e from the two-level table, e < 2^table_log.
+// Fused pass: load both operands' stage-0 words before transforming A.
+#ifndef BW_FUSED_PREFETCH
+#define BW_FUSED_PREFETCH 1
+#endif
+
 template <class AR>
 __device__ __forceinline__ typename AR::TW tw_lookup(int64_t e, const typename AR::TW* lo, const typename AR::TW* hi,
                                                      const typename AR::PC32& pc) {
@@ -417,6 +422,7 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         V keepA[KEEPN], keepB[KEEPN];
 #pragma unroll
         for (int i = 0; i < KEEPN; i++) { keepA[i] = AR::zero(); keepB[i] = AR::zero(); }
+#if BW_FUSED_PREFETCH
         IW wa[R0], wb[R0];
         load0(0, wa);
         load0(1, wb);                      // B's loads overlap A's transform
@@ -427,6 +433,15 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         fwd_stage0_from(wb);
         __syncthreads();
         fwd_stage1(1, keepB);
+#else
+        fwd_stage0(0);
+        __syncthreads();
+        fwd_stage1(0, keepA);
+        __syncthreads();
+        fwd_stage0(1);
+        __syncthreads();
+        fwd_stage1(1, keepB);
+#endif
         if (prm.fold_ninv) {
 #pragma unroll
             for (int i = 0; i < KEEPN; i++) keepA[i] = AR::mul_plain(keepA[i], keepB[i], P);

@@ -106,9 +106,9 @@ int bw_execute(bw_plan* plan, const void* d_a, const void* d_b, void* d_c, void*
  * 64-bit path: c0 = lo, c1 = mid, c2 = hi 32 bits (U160). */
 int bw_execute_crt(bw_plan* plan, const void* d_a, const void* d_b, uint64_t* d_c0, uint64_t* d_c1,
                    uint32_t* d_c2, void* stream);
-/* Launch order: BW_SCHED_PRIME_MAJOR (default) finishes one prime's chain of
- * passes before the next so intermediates are re-read from the Infinity Cache;
- * BW_SCHED_PASS_MAJOR issues one launch per pass over all primes. */
+/* Launch order: BW_SCHED_PASS_MAJOR (default) issues one launch per pass over
+ * all primes; BW_SCHED_PRIME_MAJOR finishes one prime's chain of passes before
+ * the next so intermediates are re-read from the Infinity Cache. */
 int bw_plan_set_schedule(bw_plan* plan, int schedule);
 /* Synchronizes and reports BW_ERR_STATE if a carry look-back wait of this plan
  * ever exceeded its bound (a failure that would otherwise be a hang). */
@@ -200,6 +200,39 @@ int bw_tune_count(void);
 void bw_tune_clear(void);
 /* Synchronizes; BW_ERR_STATE if a carry look-back wait exceeded its bound. */
 int bw_batch_status(bw_batch* batch);
+
+/* ---- serving (no counterpart in the reference) -----------------------------
+ * A request loop with continuous batching.  bw_server_submit() queues one
+ * product of host buffers (L_A + L_B result limbs; the buffers stay owned by
+ * the caller and must stay valid until the ticket completes; pinned memory,
+ * e.g. hipHostMalloc, lets the copies overlap the running batches).  A worker
+ * thread admits queued requests in arrival order into a batch while its device
+ * footprint fits the free part of an arena reserved at creation, and keeps up
+ * to `streams` batches in flight.  Every result is the same as the product run
+ * alone. */
+typedef struct bw_server bw_server;
+typedef struct bw_server_config {
+    uint64_t device_bytes;   /* device arena (0: 8 GiB) */
+    int streams;             /* batches in flight (0: 3, at most 8) */
+    int max_batch;           /* products per batch (0: 1024) */
+} bw_server_config;
+typedef struct bw_server_stats {
+    int64_t completed, failed, batches, batched_products, pending;
+    int max_concurrent_batches;
+    uint64_t arena_bytes, arena_peak_bytes;
+} bw_server_stats;
+int bw_server_create(bw_server** server, const bw_server_config* config);
+/* Returns a ticket (>= 0) or a negative status. */
+int64_t bw_server_submit(bw_server* server, int limb_bits, const void* h_a, uint64_t L_A, const void* h_b,
+                         uint64_t L_B, void* h_c);
+/* Blocks until the ticket completes (timeout_ms < 0: no limit); releases the
+ * ticket and reports its submit-to-completion latency. */
+int bw_server_wait(bw_server* server, int64_t ticket, int timeout_ms, double* latency_ms);
+/* 1 when the ticket has completed, 0 while it is queued or running. */
+int bw_server_poll(bw_server* server, int64_t ticket);
+int bw_server_get_stats(bw_server* server, bw_server_stats* stats);
+/* Drains queued and running requests, then frees everything. */
+void bw_server_destroy(bw_server* server);
 
 #ifdef __cplusplus
 }
