@@ -428,10 +428,13 @@ def main():
     ap.add_argument("--passes", type=str, default="", help="pass split, e.g. 12,13")
     ap.add_argument("--schedule", type=int, default=0, help="0 pass-major (default), 1 prime-major launches")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default pass split (no autotuning)")
     ap.add_argument("--no-oracle", action="store_true", help="GMP only as CPU baseline")
     ap.add_argument("--batch-n", type=int, default=4096, help="products in the batched workload (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="Poisson serving run length (0: skip)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 3 device copies of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (--gpus 1); multi-GPU runs are not part of it",
@@ -440,6 +443,16 @@ def main():
 
     import torch
     import birdwing_amd as bw
+
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy << 20
+        x = torch.zeros(n // 4, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):           # one elementwise kernel: reads n bytes, writes n bytes
+            torch.bitwise_not(x, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "device copy", "bytes_read": n, "bytes_written": n, "copies": 3}))
+        return
 
     L = args.limbs
     bits = args.bits
@@ -454,6 +467,10 @@ def main():
     plan = bw.Plan(bits, L, L, bw.SET_DEFAULT, passes)
     plan.set_schedule(args.schedule)
     stream = torch.cuda.current_stream()
+    tuned = False
+    if not passes and not args.no_tune:
+        plan.tune(da, db, stream, reps=2)      # pass split / tile cap, before any timing
+        tuned = True
 
     for _ in range(args.warmup):
         plan.execute(da, db, dc, stream)
@@ -514,7 +531,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": workload, "mode": "plain", "limbs_a": L, "limbs_b": L, "limb_bits": bits,
-                   "N": plan.N, "primes": plan.primes, "passes": plan.passes},
+                   "N": plan.N, "primes": plan.primes, "passes": plan.passes, "autotuned": tuned},
         "latency_ms": {"p50": p50, "p99": p99},
         "bit_exact_vs_gmp": exact,
         "kernels": [{"name": k, "launches": v[2], "ms_per_product": v[0], "alg_bytes": v[1],

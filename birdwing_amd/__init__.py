@@ -89,6 +89,7 @@ SIGNATURES = {
     "bw_execute_crt": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "bw_execute_host": (_i, [_vp, _vp, _vp, _vp]),
     "bw_plan_status": (_i, [_vp]),
+    "bw_plan_tune": (_i, [_vp, _vp, _vp, _vp, _i]),
     "bw_plan_set_schedule": (_i, [_vp, _i]),
     "bw_plan_set_timing": (_i, [_vp, _i]),
     "bw_plan_timing": (_i, [_vp, C.POINTER(C.c_float), _i, C.POINTER(_i)]),
@@ -271,6 +272,14 @@ class Plan:
         order = np.zeros(self.N_exec, dtype=np.uint64)
         _check(lib().bw_spectrum_order(self._p, order.ctypes.data_as(C.POINTER(C.c_uint64))))
         return order
+
+    def tune(self, a, b, stream=None, reps=3):
+        """Autotune the pass split / tile cap on device operands a, b."""
+        _check(lib().bw_plan_tune(self._p, _ptr(a), _ptr(b), _stream(stream), reps))
+        info = PlanInfo()
+        _check(lib().bw_plan_get_info(self._p, C.byref(info)))
+        self.info = info
+        self.passes = [int(info.pass_log[i]) for i in range(info.npass)]
 
     def set_schedule(self, schedule):
         """0: one launch per pass over all primes; 1: per-prime chains (default)."""

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/birdwing.h"
+#include "batch.h"
 #include "engine.h"
 #include "misc_kernels.h"
 
@@ -131,6 +132,7 @@ struct bw_plan {
     void* h_c = nullptr;
     hipStream_t own = nullptr;
     int schedule = BW_SCHED_PASS_MAJOR;
+    int gshift = 0;
     // timing: one event set (launches + 1 events) per recorded execution
     bool timing = false;
     std::vector<std::vector<hipEvent_t>> evpool;
@@ -204,7 +206,13 @@ extern "C" int bw_plan_create_ex(bw_plan** out, int limb_bits, uint64_t L_A, uin
             return fail(BW_ERR_ARG, "pass split does not add up to log2 N = %d (or more than 4 passes)", logN);
         }
     } else {
-        default_passes(logN, p->passes);
+        TuneChoice c;
+        if (tune_lookup(TuneKey{limb_bits, logN, 0}, &c)) {   // autotuned split for this size
+            p->passes = c.passes;
+            p->gshift = c.gshift;
+        } else {
+            default_passes(logN, p->passes);
+        }
     }
     int rc = get_tables(set_id, &p->tabs);
     if (rc) { delete p; return rc; }
@@ -254,6 +262,7 @@ static Xform plan_xform(const bw_plan* p) {
     x.bufB = p->bufB;
     x.ninv = p->ninv;
     x.schedule = p->schedule;
+    x.gshift = p->gshift;
     return x;
 }
 
@@ -381,6 +390,61 @@ extern "C" int bw_execute_host(bw_plan* p, const void* a, const void* b, void* c
     if ((rc = bw_execute(p, p->h_a, p->h_b, p->h_c, p->own))) return rc;
     HIP_TRY(hipMemcpyAsync(c, p->h_c, (p->L_A + p->L_B) * wb, hipMemcpyDeviceToHost, p->own));
     HIP_TRY(hipStreamSynchronize(p->own));
+    return BW_OK;
+}
+
+extern "C" int bw_plan_tune(bw_plan* p, const void* d_a, const void* d_b, void* stream, int reps) {
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!d_a || !d_b) return fail(BW_ERR_ARG, "NULL device buffer");
+    if (reps < 1) reps = 3;
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<TuneChoice> cands;
+    tune_candidates(p->logN, cands);
+    {   // the current choice competes too
+        TuneChoice cur;
+        cur.passes = p->passes;
+        cur.gshift = p->gshift;
+        cands.insert(cands.begin(), cur);
+    }
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    TuneChoice best;
+    best.ms = -1.0f;
+    for (const auto& c : cands) {
+        bool reg2 = true;
+        for (int t : c.passes) reg2 = reg2 && t <= BW_REG2_MAX_LOG;
+        if (!reg2 && c.passes != p->passes) continue;
+        Xform x = plan_xform(p);
+        x.passes = c.passes;
+        x.gshift = c.gshift;
+        float ms_min = -1.0f;
+        for (int r = 0; r < reps + 1 && !rc; r++) {
+            if (hipEventRecord(e0, st) != hipSuccess) { rc = fail(BW_ERR_HIP, "hipEventRecord"); break; }
+            if ((rc = run_transforms(x, d_a, d_b, (int64_t)p->L_A, (int64_t)p->L_B, st, nullptr))) break;
+            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                rc = fail(BW_ERR_HIP, "tuning event");
+                break;
+            }
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (r > 0 && (ms_min < 0 || ms < ms_min)) ms_min = ms;    // first run warms up
+        }
+        if (rc) break;
+        if (best.ms < 0 || ms_min < best.ms) {
+            best = c;
+            best.ms = ms_min;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (best.ms >= 0) {
+        p->passes = best.passes;
+        p->gshift = best.gshift;
+        tune_store(TuneKey{p->limb_bits, p->logN, 0}, best);
+    }
     return BW_OK;
 }
 

@@ -37,8 +37,8 @@ void tune_store(const TuneKey& k, const TuneChoice& c) {
 
 // Slots of a one-pass group are padded to the rows tile so that every
 // workgroup holds whole products.
-static int64_t rows_tile(int logN, int gshift) {
-    int g = bw_pass_max_G(logN) >> gshift;
+static int64_t rows_tile(int logN, int limb_bits, int gshift) {
+    int g = bw_pass_max_G(logN, limb_bits) >> gshift;
     if (g < 1) g = 1;
     while (g > 1 && ((int64_t)g << logN) > 4096) g >>= 1;
     return g;
@@ -48,7 +48,7 @@ static void group_geometry(BatchGroup& g) {
     const int64_t m = (int64_t)g.members.size();
     g.B = m;
     if (g.passes.size() == 1) {
-        const int64_t t = rows_tile(g.logN, g.gshift);
+        const int64_t t = rows_tile(g.logN, g.limb_bits, g.gshift);
         g.B = (m + t - 1) / t * t;
     }
 }
@@ -446,8 +446,9 @@ extern "C" int bw_batch_status(bw_batch* b) {
     return BW_OK;
 }
 
-// Candidate (pass split, tile cap) pairs for a group of 2^logN points.
-static void tune_candidates(int logN, std::vector<TuneChoice>& out) {
+namespace bw {
+// Candidate (pass split, tile cap) pairs for transforms of 2^logN points.
+void tune_candidates(int logN, std::vector<TuneChoice>& out) {
     out.clear();
     auto add = [&](std::vector<int> p) {
         for (int gs = 0; gs <= 2; gs++) {
@@ -471,6 +472,7 @@ static void tune_candidates(int logN, std::vector<TuneChoice>& out) {
             }
     }
 }
+}  // namespace bw
 
 extern "C" int bw_batch_tune(bw_batch* b, const void* d_a, const void* d_b, void* stream, int reps) {
     if (!b) return fail(BW_ERR_ARG, "batch is NULL");

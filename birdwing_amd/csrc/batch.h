@@ -67,6 +67,8 @@ struct TuneChoice {
 bool tune_lookup(const TuneKey& k, TuneChoice* c);
 void tune_store(const TuneKey& k, const TuneChoice& c);
 int log2_bucket(int64_t B);
+// Candidate (pass split, tile cap) pairs for transforms of 2^logN points.
+void tune_candidates(int logN, std::vector<TuneChoice>& out);
 
 // Builds the layout (and validates every size).  Returns BW_OK or an error.
 int batch_layout(int n, const int* bits, const uint64_t* LA, const uint64_t* LB, BatchLayout& L);

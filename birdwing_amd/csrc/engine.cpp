@@ -68,17 +68,6 @@ void default_passes(int logN, std::vector<int>& out) {
     const int P = (logN + 8) / 9 < 1 ? 1 : (logN + 8) / 9;
     const int base = logN / P, extra = logN % P;
     for (int j = 0; j < P; j++) out.push_back(base + (j >= P - extra ? 1 : 0));
-    // the fused pass holds two operands' stage-1 values in registers: past
-    // 2^8 points it loses occupancy, so its excess moves to a middle pass
-    // (or, with two passes, to the first)
-    while (P > 1 && out[P - 1] > 8) {
-        int best = P > 2 ? 1 : 0;
-        for (int j = 1; j < P - 1; j++)
-            if (out[j] < out[best]) best = j;
-        if (out[best] >= BW_REG2_MAX_LOG) break;
-        out[best]++;
-        out[P - 1]--;
-    }
 }
 
 PassPlan pass_geom(const Xform& x, int j) {
@@ -89,7 +78,7 @@ PassPlan pass_geom(const Xform& x, int j) {
     g.logM = x.logN - prefix;
     g.S = (int64_t)1 << (x.logN - prefix - g.logT);
     g.O = x.B << prefix;
-    int gmax = bw_pass_max_G(g.logT) >> x.gshift;
+    int gmax = bw_pass_max_G(g.logT, x.limb_bits) >> x.gshift;
     if (gmax < 1) gmax = 1;
     if (g.S > 1) {
         g.G = (int)(g.S < gmax ? g.S : gmax);

@@ -68,9 +68,8 @@ int run_transforms(const Xform& x, const void* d_a, const void* d_b, int64_t la,
                    LaunchMarker* mk);
 int count_transform_launches(const Xform& x);
 
-// Default split of 2^logN into passes: as balanced as possible with at most
-// 2^9 points each, except that the last (fused) pass keeps at most 2^8 points
-// and the first (ingress) pass is not enlarged while a middle pass can be.
+// Default split of 2^logN into passes of at most 2^9 points, as balanced as
+// possible, the larger passes last (the measured best at 2^25: 8, 8, 9).
 void default_passes(int logN, std::vector<int>& out);
 
 }  // namespace bw

@@ -326,24 +326,41 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
 #pragma unroll
         for (int k = 0; k < R0; k++) lds[g0 * LT + T2Lds<LOGT>::pos(k * R1 + j0)] = AR::to_lds(x[brev(k, R0LOG)], P);
     };
+    // fused pass: A's stage-1 spectrum, then the pointwise product, held in registers
+    constexpr int KEEPN = IT1 * R1;
+    V keep[KEEPN];
+#pragma unroll
+    for (int i = 0; i < KEEPN; i++) keep[i] = AR::zero();
     auto dec1 = [&](int gi, int& g, int& b) {
         if (COLS) { g = gi & (G - 1); b = gi >> logG; } else { g = gi >> R0LOG; b = gi & (R0 - 1); }
     };
     // stage 1 forward: LDS -> DFT_R1 -> keep (fused) or pass twiddle -> HBM
-    auto fwd_stage1 = [&](int which, V* keep) {
+    // KM (compile-time): 0 store to HBM with the pass twiddle, 1 keep[i] = x[i],
+    // 2 keep[i] *= x[i], 3 keep[i] *= x[i] * N^-1 (the pointwise product)
+    auto fwd_stage1 = [&](int which, auto km) {
+        constexpr int KM = decltype(km)::value;
 #pragma unroll
         for (int it = 0; it < IT1; it++) {
             const int gi = tid + it * NT;
-            if (gi >= G * R0) break;
+            if (IT1 * NT > G * R0 && gi >= G * R0) continue;   // idle stage-1 slot
             int g, b;
             dec1(gi, g, b);
             V x[R1];
 #pragma unroll
             for (int m = 0; m < R1; m++) x[m] = AR::from_lds(lds[g * LT + T2Lds<LOGT>::pos(b * R1 + m)]);
             dft32_dif<AR, R1LOG>(x, pc);
-            if (keep) {
+            if constexpr (KM == 1) {
 #pragma unroll
                 for (int k = 0; k < R1; k++) keep[it * R1 + k] = x[brev(k, R1LOG)];
+                continue;
+            } else if constexpr (KM == 2) {
+#pragma unroll
+                for (int k = 0; k < R1; k++) keep[it * R1 + k] = AR::mul_plain(keep[it * R1 + k], x[brev(k, R1LOG)], P);
+                continue;
+            } else if constexpr (KM == 3) {
+#pragma unroll
+                for (int k = 0; k < R1; k++)
+                    keep[it * R1 + k] = AR::mul_scaled(keep[it * R1 + k], x[brev(k, R1LOG)], ninv, P);
                 continue;
             }
             W* out = reinterpret_cast<W*>(prm.out[which]) + plane_off + line_base(g) + (int64_t)b * R1 * tstride;
@@ -361,15 +378,16 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         }
     };
     // stage 1 inverse: keep (fused) or HBM with the inverse pass twiddle -> inverse DFT_R1 -> LDS
-    auto inv_stage1 = [&](const V* keep) {
+    auto inv_stage1 = [&](auto km) {
+        constexpr bool KEEP = decltype(km)::value != 0;
 #pragma unroll
         for (int it = 0; it < IT1; it++) {
             const int gi = tid + it * NT;
-            if (gi >= G * R0) break;
+            if (IT1 * NT > G * R0 && gi >= G * R0) continue;   // idle stage-1 slot
             int g, b;
             dec1(gi, g, b);
             V x[R1];
-            if (keep) {
+            if constexpr (KEEP) {
 #pragma unroll
                 for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = keep[it * R1 + k];
             } else {
@@ -412,45 +430,34 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     if constexpr (MODE == BW_MODE_FWD) {
         fwd_stage0(op);
         __syncthreads();
-        fwd_stage1(op, nullptr);
+        fwd_stage1(op, std::integral_constant<int, 0>());
     } else if constexpr (MODE == BW_MODE_INV) {
-        inv_stage1(nullptr);
+        inv_stage1(std::integral_constant<int, 0>());
         __syncthreads();
         inv_stage0();
     } else {
-        constexpr int KEEPN = IT1 * R1;
-        V keepA[KEEPN], keepB[KEEPN];
-#pragma unroll
-        for (int i = 0; i < KEEPN; i++) { keepA[i] = AR::zero(); keepB[i] = AR::zero(); }
 #if BW_FUSED_PREFETCH
         IW wa[R0], wb[R0];
         load0(0, wa);
         load0(1, wb);                      // B's loads overlap A's transform
         fwd_stage0_from(wa);
-        __syncthreads();
-        fwd_stage1(0, keepA);
-        __syncthreads();
-        fwd_stage0_from(wb);
-        __syncthreads();
-        fwd_stage1(1, keepB);
 #else
         fwd_stage0(0);
-        __syncthreads();
-        fwd_stage1(0, keepA);
-        __syncthreads();
-        fwd_stage0(1);
-        __syncthreads();
-        fwd_stage1(1, keepB);
 #endif
-        if (prm.fold_ninv) {
-#pragma unroll
-            for (int i = 0; i < KEEPN; i++) keepA[i] = AR::mul_plain(keepA[i], keepB[i], P);
-        } else {
-#pragma unroll
-            for (int i = 0; i < KEEPN; i++) keepA[i] = AR::mul_scaled(keepA[i], keepB[i], ninv, P);
-        }
         __syncthreads();
-        inv_stage1(keepA);
+        fwd_stage1(0, std::integral_constant<int, 1>());
+        __syncthreads();
+#if BW_FUSED_PREFETCH
+        fwd_stage0_from(wb);
+#else
+        fwd_stage0(1);
+#endif
+        __syncthreads();
+        // B's spectrum multiplies into A's
+        if (prm.fold_ninv) fwd_stage1(1, std::integral_constant<int, 2>());
+        else fwd_stage1(1, std::integral_constant<int, 3>());
+        __syncthreads();
+        inv_stage1(std::integral_constant<int, 1>());
         __syncthreads();
         inv_stage0();
     }

@@ -56,14 +56,15 @@ struct PassLaunch {
 };
 hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_t stream);
 // Largest lines-per-workgroup the kernel set provides for a pass of 2^logT points.
-int bw_pass_max_G(int logT);
+int bw_pass_max_G(int logT, int limb_bits);
 // Passes of up to 2^BW_REG2_MAX_LOG points use the register-staged kernels
 // (ntt2_impl.h), larger ones the LDS radix-16 kernels (ntt_pass_impl.h).
 #define BW_REG2_MAX_LOG 10
 // Radix split T = R0 * R1 of the register-staged kernels: R1 (stage 1,
-// stride 1) takes up to 16 points below T = 2^9 and 32 points from 2^9 on, so
-// the stage-0 radix R0 (which sets the LDS bytes per wavefront) stays <= 32.
-__host__ __device__ constexpr int t2_r1log(int logT) { return logT <= 5 ? logT : (logT <= 8 ? 4 : 5); }
+// stride 1) takes up to 16 points through T = 2^9 and 32 points at 2^10, so
+// the stage-0 radix R0 stays <= 32 and a 2^9-point column pass still runs 32
+// adjacent columns (128-byte rows) in 512 threads.
+__host__ __device__ constexpr int t2_r1log(int logT) { return logT <= 5 ? logT : (logT <= 9 ? 4 : 5); }
 __host__ __device__ constexpr int t2_r0log(int logT) { return logT - t2_r1log(logT); }
 int bw_reg2_lds_line(int logT);
 // Frequency held at position q of a pass line (either kernel family).

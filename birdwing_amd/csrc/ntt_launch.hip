@@ -18,12 +18,14 @@ PassKernel bw_pick2_u64_fwd(int logT, bool cols, bool ingress);
 PassKernel bw_pick2_u64_inv(int logT, bool cols, bool ingress);
 PassKernel bw_pick2_u64_fused(int logT, bool cols, bool ingress);
 
-int bw_pass_max_G(int logT) {
+int bw_pass_max_G(int logT, int limb_bits) {
     if (logT <= BW_REG2_MAX_LOG) {
-        // register-staged: G * T <= 8192 (LDS) and G * R1 <= 512 threads
+        // register-staged: G * R1 <= 512 threads and at most 64 KiB of LDS
+        // values (4-byte values on the 32-bit path, 8-byte on the 64-bit path)
         const int r1 = t2_r1log(logT);
+        const int cap = limb_bits == 64 ? 8192 : 16384;
         int g = 512 >> r1;
-        while (g > 1 && (1 << logT) * g > 8192) g >>= 1;
+        while (g > 1 && (1 << logT) * g > cap) g >>= 1;
         return g;
     }
     int g = 16;
@@ -59,7 +61,7 @@ hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_
     // LDS element: 8 bytes, except the register-staged 32-bit path (centered int32)
     const size_t vbytes = (reg2 && L.limb_bits == 32) ? 4 : 8;
     const size_t lds = (size_t)L.G * (reg2 ? bw_reg2_lds_line(L.logT) : lds_line(L.logT)) * vbytes;
-    if (L.G > bw_pass_max_G(L.logT) || (L.G & (L.G - 1))) return hipErrorInvalidValue;
+    if (L.G > bw_pass_max_G(L.logT, L.limb_bits) || (L.G & (L.G - 1))) return hipErrorInvalidValue;
     if (prm.bdesc && prm.ingress && !reg2) return hipErrorInvalidValue;   // stacked ingress: reg2 only
     {
         // Allow the largest tile this kernel is ever launched with, once per kernel.
@@ -67,7 +69,7 @@ hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_
         static std::unordered_set<const void*> done;
         std::lock_guard<std::mutex> lock(mu);
         if (!done.count(reinterpret_cast<const void*>(k))) {
-            const size_t lds_max = (size_t)bw_pass_max_G(L.logT) *
+            const size_t lds_max = (size_t)bw_pass_max_G(L.logT, L.limb_bits) *
                                    (reg2 ? bw_reg2_lds_line(L.logT) : lds_line(L.logT)) * vbytes;
             hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max);

@@ -110,6 +110,11 @@ int bw_execute_crt(bw_plan* plan, const void* d_a, const void* d_b, uint64_t* d_
  * all primes; BW_SCHED_PRIME_MAJOR finishes one prime's chain of passes before
  * the next so intermediates are re-read from the Infinity Cache. */
 int bw_plan_set_schedule(bw_plan* plan, int schedule);
+/* Autotuner: times candidate pass splits and tile caps of this plan's
+ * transforms on the given operands (reps timed runs each, synchronizing),
+ * keeps the fastest, and records it in the process-wide table that later
+ * plans of the same limb width and size start from. */
+int bw_plan_tune(bw_plan* plan, const void* d_a, const void* d_b, void* stream, int reps);
 /* Synchronizes and reports BW_ERR_STATE if a carry look-back wait of this plan
  * ever exceeded its bound (a failure that would otherwise be a hang). */
 int bw_plan_status(bw_plan* plan);

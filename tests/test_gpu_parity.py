@@ -375,3 +375,20 @@ def test_unsupported_sizes_raise(gpu):
     assert e.value.code == gpu.BW_ERR_UNSUPPORTED_SIZE
     with pytest.raises(gpu.BWError):
         gpu.Plan(32, 0, 5)
+
+
+@pytest.mark.parametrize("bits,L", [(32, 1 << 15), (64, 5000)])
+def test_plan_autotune_keeps_product(gpu, oracle, bits, L):
+    """The autotuned pass split computes the same product (vs GMP)."""
+    A = oracle.uniform_limbs(L, 501, bits)
+    B = oracle.uniform_limbs(L + 3, 502, bits)
+    d = dev()
+    plan = gpu.Plan(bits, len(A), len(B))
+    da, db = d.to_dev(A), d.to_dev(B)
+    plan.tune(da, db, reps=1)
+    c = d.empty(len(A) + len(B), bits)
+    plan.execute(da, db, c)
+    want, _ = oracle.gmp_mul(A, B, bits)
+    assert np.array_equal(d.from_dev(c, len(A) + len(B)), want), plan.passes
+    # a new plan of the same shape starts from the tuned split
+    assert gpu.Plan(bits, len(A), len(B)).passes == plan.passes
