@@ -568,7 +568,7 @@ extern "C" int bw_spectrum_order(const bw_plan* p, uint64_t* order) {
         uint64_t f = 0, mult = 1;
         for (int j = 0; j < P; j++) {
             const int64_t qj = (int64_t)(q / S[j]) & (((int64_t)1 << p->passes[j]) - 1);
-            f += (uint64_t)bw_pass_freq_of_pos(p->passes[j], qj) * mult;
+            f += (uint64_t)bw_pass_freq_of_pos(p->passes[j], qj, j < P - 1) * mult;
             mult <<= p->passes[j];
         }
         order[q] = f;

@@ -38,7 +38,7 @@ void tune_store(const TuneKey& k, const TuneChoice& c) {
 // Slots of a one-pass group are padded to the rows tile so that every
 // workgroup holds whole products.
 static int64_t rows_tile(int logN, int limb_bits, int gshift) {
-    int g = bw_pass_max_G(logN, limb_bits) >> gshift;
+    int g = bw_pass_max_G(logN, limb_bits, false) >> gshift;
     if (g < 1) g = 1;
     while (g > 1 && ((int64_t)g << logN) > 4096) g >>= 1;
     return g;

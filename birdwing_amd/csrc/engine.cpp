@@ -78,7 +78,7 @@ PassPlan pass_geom(const Xform& x, int j) {
     g.logM = x.logN - prefix;
     g.S = (int64_t)1 << (x.logN - prefix - g.logT);
     g.O = x.B << prefix;
-    int gmax = bw_pass_max_G(g.logT, x.limb_bits) >> x.gshift;
+    int gmax = bw_pass_max_G(g.logT, x.limb_bits, g.S > 1) >> x.gshift;
     if (gmax < 1) gmax = 1;
     if (g.S > 1) {
         g.G = (int)(g.S < gmax ? g.S : gmax);

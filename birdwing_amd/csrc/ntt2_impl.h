@@ -37,6 +37,10 @@ struct ArF64x {
     __device__ static W to_word_c(V x, const PR& P, bool canon) {
         return canon ? f64_canon(x, P) : (uint32_t)(int32_t)f64_reduce(x, P);
     }
+    // values already in (-p, p) -- outputs of a modular product -- need no
+    // reduction before the 32-bit conversion
+    __device__ static LV to_lds_r(V x) { return (int32_t)x; }
+    __device__ static W to_word_r(V x) { return (uint32_t)(int32_t)x; }
     // a*b (no N^-1) for a, b < 2^46 in magnitude
     __device__ static V mul_plain(V a, V b, const PR& P) {
         const double br = f64_reduce(b, P);
@@ -100,6 +104,8 @@ struct ArU64x {
     __device__ static V from_raw(Raw x, const PR& P) { return u64_reduce(x, P); }
     __device__ static W to_word(V x, const PR&) { return x; }
     __device__ static W to_word_c(V x, const PR&, bool) { return x; }
+    __device__ static LV to_lds_r(V x) { return x; }
+    __device__ static W to_word_r(V x) { return x; }
     // montmul(montmul(a, b), 2^128) = a*b
     __device__ static V mul_plain(V a, V b, const PR& P) { return u64_montmul(u64_montmul(a, b, P), P.r2, P); }
     // ninv[0] = N^-1 2^128 mod p: montmul(t, ninv) = t N^-1 2^64 -> montmul by 1 -> t N^-1
@@ -189,13 +195,12 @@ __device__ __forceinline__ void dft32_dit_inv(typename AR::V* x, const typename 
 
 
 // LDS line: pad one slot per R1 elements, odd line stride.
-template <int LOGT>
+template <int LOGT, bool COLS>
 struct T2Lds {
-    static constexpr int R1 = 1 << t2_r1log(LOGT);
+    static constexpr int R1 = 1 << t2_r1log(LOGT, COLS);
     static constexpr int LT = (1 << LOGT) + (1 << LOGT) / R1 + 1;
     __device__ static int pos(int q) { return q + q / R1; }
 };
-
 // w_M^e from the two-level table, e < 2^table_log.
 // Fused pass: load both operands' stage-0 words before transforming A.
 #ifndef BW_FUSED_PREFETCH
@@ -220,9 +225,9 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     using W = typename AR::W;
     using TW = typename AR::TW;
     constexpr int T = 1 << LOGT;
-    constexpr int R1LOG = t2_r1log(LOGT), R0LOG = t2_r0log(LOGT);
+    constexpr int R1LOG = t2_r1log(LOGT, COLS), R0LOG = t2_r0log(LOGT, COLS);
     constexpr int R1 = 1 << R1LOG, R0 = 1 << R0LOG;
-    constexpr int LT = T2Lds<LOGT>::LT;
+    constexpr int LT = t2_lds_line(LOGT, COLS);   // the host sizes LDS with the same function
     constexpr int IT1 = R0 > R1 ? R0 / R1 : 1;     // stage-1 groups per thread (upper bound)
     constexpr int TWSH = BW_STAGE_LOG - LOGT;      // w_T = w_8192^(2^TWSH)
     constexpr int TWMASK = (1 << BW_STAGE_LOG) - 1;
@@ -232,15 +237,15 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     const int NT = blockDim.x;
     const int logG = __builtin_ctz(NT) - R1LOG;
     const int G = 1 << logG;
-    const int prime = prm.prime0 + blockIdx.y;
     const int op = (MODE == BW_MODE_FWD) ? blockIdx.z : 0;
+    const int bx = blockIdx.x, prime = prm.prime0 + blockIdx.y;
     const typename AR::PC32 pc = reinterpret_cast<const typename AR::PC32*>(prm.consts)[prime];
     const auto& P = AR::prime(pc);
     const TW* tws = reinterpret_cast<const TW*>(prm.tw_stage) + ((int64_t)prime << BW_STAGE_LOG);
     const TW* twlo = reinterpret_cast<const TW*>(prm.tw_lo) + ((int64_t)prime << BW_TW_LO_LOG);
     const TW* twhi = reinterpret_cast<const TW*>(prm.tw_hi) + ((int64_t)prime << (prm.table_log - BW_TW_LO_LOG));
     const int64_t S = prm.S;
-    const int64_t l0 = (int64_t)blockIdx.x << logG;
+    const int64_t l0 = (int64_t)bx << logG;
     const int64_t o0 = COLS ? l0 / S : l0;
     const int64_t s0 = COLS ? l0 - o0 * S : 0;
     const int twshift = prm.table_log - prm.logM;
@@ -300,7 +305,9 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
 #pragma unroll
         for (int k = 1; k < R0; k++) x[brev(k, R0LOG)] = AR::mul(x[brev(k, R0LOG)], tws[(j0 * k) << TWSH], P);
 #pragma unroll
-        for (int k = 0; k < R0; k++) lds[g0 * LT + T2Lds<LOGT>::pos(k * R1 + j0)] = AR::to_lds(x[brev(k, R0LOG)], P);
+        for (int k = 0; k < R0; k++)   // k > 0 left a twiddle product in (-p, p)
+            lds[g0 * LT + T2Lds<LOGT, COLS>::pos(k * R1 + j0)] =
+                k == 0 ? AR::to_lds(x[0], P) : AR::to_lds_r(x[brev(k, R0LOG)]);
     };
     auto fwd_stage0 = [&](int which) {
         int64_t base = line_base(g0) + j0 * tstride;
@@ -324,7 +331,9 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
 #pragma unroll
         for (int k = 1; k < R0; k++) x[brev(k, R0LOG)] = AR::mul(x[brev(k, R0LOG)], tws[(j0 * k) << TWSH], P);
 #pragma unroll
-        for (int k = 0; k < R0; k++) lds[g0 * LT + T2Lds<LOGT>::pos(k * R1 + j0)] = AR::to_lds(x[brev(k, R0LOG)], P);
+        for (int k = 0; k < R0; k++)   // k > 0 left a twiddle product in (-p, p)
+            lds[g0 * LT + T2Lds<LOGT, COLS>::pos(k * R1 + j0)] =
+                k == 0 ? AR::to_lds(x[0], P) : AR::to_lds_r(x[brev(k, R0LOG)]);
     };
     // fused pass: A's stage-1 spectrum, then the pointwise product, held in registers
     constexpr int KEEPN = IT1 * R1;
@@ -347,7 +356,7 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
             dec1(gi, g, b);
             V x[R1];
 #pragma unroll
-            for (int m = 0; m < R1; m++) x[m] = AR::from_lds(lds[g * LT + T2Lds<LOGT>::pos(b * R1 + m)]);
+            for (int m = 0; m < R1; m++) x[m] = AR::from_lds(lds[g * LT + T2Lds<LOGT, COLS>::pos(b * R1 + m)]);
             dft32_dif<AR, R1LOG>(x, pc);
             if constexpr (KM == 1) {
 #pragma unroll
@@ -370,10 +379,18 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
                 AR::template gen_powers<R1>(tw_lookup<AR>(((int64_t)b * s) << twshift, twlo, twhi, pc),
                                             tw_lookup<AR>(((int64_t)R0 * s) << twshift, twlo, twhi, pc), tw, P);
 #pragma unroll
-                for (int k = 0; k < R1; k++) out[k * tstride] = AR::to_word_c(AR::apply_tw(x[brev(k, R1LOG)], tw[k], P), P, canon);
+                for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::apply_tw(x[brev(k, R1LOG)], tw[k], P);
+            }
+            // one uniform branch around the stores: no per-element select of both forms
+            if (canon) {
+#pragma unroll
+                for (int k = 0; k < R1; k++) out[k * tstride] = AR::to_word_c(x[brev(k, R1LOG)], P, true);
+            } else if constexpr (COLS) {      // every value left a pass-twiddle product in (-p, p)
+#pragma unroll
+                for (int k = 0; k < R1; k++) out[k * tstride] = AR::to_word_r(x[brev(k, R1LOG)]);
             } else {
 #pragma unroll
-                for (int k = 0; k < R1; k++) out[k] = AR::to_word_c(x[brev(k, R1LOG)], P, canon);
+                for (int k = 0; k < R1; k++) out[k * tstride] = AR::to_word_c(x[brev(k, R1LOG)], P, false);
             }
         }
     };
@@ -409,22 +426,27 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
             }
             dft32_dit_inv<AR, R1LOG>(x, pc);
 #pragma unroll
-            for (int m = 0; m < R1; m++) lds[g * LT + T2Lds<LOGT>::pos(b * R1 + m)] = AR::to_lds(x[m], P);
+            for (int m = 0; m < R1; m++) lds[g * LT + T2Lds<LOGT, COLS>::pos(b * R1 + m)] = AR::to_lds(x[m], P);
         }
     };
     // stage 0 inverse: LDS -> inverse stage twiddle -> inverse DFT_R0 -> HBM
     auto inv_stage0 = [&]() {
         V x[R0];
 #pragma unroll
-        for (int k = 0; k < R0; k++) x[brev(k, R0LOG)] = AR::from_lds(lds[g0 * LT + T2Lds<LOGT>::pos(k * R1 + j0)]);
+        for (int k = 0; k < R0; k++) x[brev(k, R0LOG)] = AR::from_lds(lds[g0 * LT + T2Lds<LOGT, COLS>::pos(k * R1 + j0)]);
 #pragma unroll
         for (int k = 1; k < R0; k++)
             x[brev(k, R0LOG)] = AR::mul(x[brev(k, R0LOG)], tws[(-((j0 * k) << TWSH)) & TWMASK], P);
         dft32_dit_inv<AR, R0LOG>(x, pc);
         W* out = reinterpret_cast<W*>(prm.out[0]) + plane_off + line_base(g0) + j0 * tstride;
         const int64_t step = (int64_t)R1 * tstride;
+        if (canon) {
 #pragma unroll
-        for (int m = 0; m < R0; m++) out[m * step] = AR::to_word_c(x[m], P, canon);
+            for (int m = 0; m < R0; m++) out[m * step] = AR::to_word_c(x[m], P, true);
+        } else {
+#pragma unroll
+            for (int m = 0; m < R0; m++) out[m * step] = AR::to_word_c(x[m], P, false);
+        }
     };
 
     if constexpr (MODE == BW_MODE_FWD) {
@@ -436,22 +458,23 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         __syncthreads();
         inv_stage0();
     } else {
-#if BW_FUSED_PREFETCH
-        IW wa[R0], wb[R0];
-        load0(0, wa);
-        load0(1, wb);                      // B's loads overlap A's transform
-        fwd_stage0_from(wa);
-#else
-        fwd_stage0(0);
-#endif
+        // prefetch B's stage-0 words while A is transformed, unless the two
+        // register sets would cost occupancy (radix-32 stage 0)
+        constexpr bool PREFETCH = BW_FUSED_PREFETCH && R0 <= 16;
+        IW wb[PREFETCH ? R0 : 1];
+        if constexpr (PREFETCH) {
+            IW wa[R0];
+            load0(0, wa);
+            load0(1, wb);
+            fwd_stage0_from(wa);
+        } else {
+            fwd_stage0(0);
+        }
         __syncthreads();
         fwd_stage1(0, std::integral_constant<int, 1>());
         __syncthreads();
-#if BW_FUSED_PREFETCH
-        fwd_stage0_from(wb);
-#else
-        fwd_stage0(1);
-#endif
+        if constexpr (PREFETCH) fwd_stage0_from(wb);
+        else fwd_stage0(1);
         __syncthreads();
         // B's spectrum multiplies into A's
         if (prm.fold_ninv) fwd_stage1(1, std::integral_constant<int, 2>());

@@ -56,21 +56,33 @@ struct PassLaunch {
 };
 hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_t stream);
 // Largest lines-per-workgroup the kernel set provides for a pass of 2^logT points.
-int bw_pass_max_G(int logT, int limb_bits);
+int bw_pass_max_G(int logT, int limb_bits, bool cols);
 // Passes of up to 2^BW_REG2_MAX_LOG points use the register-staged kernels
 // (ntt2_impl.h), larger ones the LDS radix-16 kernels (ntt_pass_impl.h).
 #define BW_REG2_MAX_LOG 10
 // Radix split T = R0 * R1 of the register-staged kernels: R1 (stage 1,
-// stride 1) takes up to 16 points through T = 2^9 and 32 points at 2^10, so
-// the stage-0 radix R0 stays <= 32 and a 2^9-point column pass still runs 32
-// adjacent columns (128-byte rows) in 512 threads.
-__host__ __device__ constexpr int t2_r1log(int logT) { return logT <= 5 ? logT : (logT <= 9 ? 4 : 5); }
-__host__ __device__ constexpr int t2_r0log(int logT) { return logT - t2_r1log(logT); }
-int bw_reg2_lds_line(int logT);
+// stride 1) takes up to 16 points and 32 above, so the stage-0 radix R0 stays
+// <= 32.  Column passes keep R1 = 16 through T = 2^9, so a 2^9-point column
+// pass still runs 32 adjacent columns (128-byte rows) in 512 threads; row
+// passes switch to R1 = 32 at 2^9, which keeps the fused pass's stage-0
+// registers at radix 16.
+__host__ __device__ constexpr int t2_r1log(int logT, bool cols) {
+    return logT <= 5 ? logT : (logT <= (cols ? 9 : 8) ? 4 : 5);
+}
+__host__ __device__ constexpr int t2_r0log(int logT, bool cols) { return logT - t2_r1log(logT, cols); }
+// LDS line stride (values) of a tile line: odd for column tiles (adjacent
+// threads own adjacent lines); for row tiles with 16-wide stage-1 groups,
+// 16 mod 32 so the two 16-thread groups of a half-wavefront hit disjoint banks.
+__host__ __device__ constexpr int t2_lds_line(int logT, bool cols) {
+    return (!cols && t2_r1log(logT, cols) == 4)
+               ? ((1 << logT) + ((1 << logT) >> 4) + 1 + 15) / 32 * 32 + 16
+               : (1 << logT) + ((1 << logT) >> t2_r1log(logT, cols)) + 1;
+}
+int bw_reg2_lds_line(int logT, bool cols);
 // Frequency held at position q of a pass line (either kernel family).
-__host__ __device__ inline int64_t bw_pass_freq_of_pos(int logT, int64_t q) {
+__host__ __device__ inline int64_t bw_pass_freq_of_pos(int logT, int64_t q, bool cols) {
     if (logT <= BW_REG2_MAX_LOG) {
-        const int r1l = t2_r1log(logT);
+        const int r1l = t2_r1log(logT, cols);
         const int r0l = logT - r1l;
         return (q >> r1l) + ((q & ((1 << r1l) - 1)) << r0l);
     }

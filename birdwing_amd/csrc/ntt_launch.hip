@@ -18,11 +18,11 @@ PassKernel bw_pick2_u64_fwd(int logT, bool cols, bool ingress);
 PassKernel bw_pick2_u64_inv(int logT, bool cols, bool ingress);
 PassKernel bw_pick2_u64_fused(int logT, bool cols, bool ingress);
 
-int bw_pass_max_G(int logT, int limb_bits) {
+int bw_pass_max_G(int logT, int limb_bits, bool cols) {
     if (logT <= BW_REG2_MAX_LOG) {
         // register-staged: G * R1 <= 512 threads and at most 64 KiB of LDS
         // values (4-byte values on the 32-bit path, 8-byte on the 64-bit path)
-        const int r1 = t2_r1log(logT);
+        const int r1 = t2_r1log(logT, cols);
         const int cap = limb_bits == 64 ? 8192 : 16384;
         int g = 512 >> r1;
         while (g > 1 && (1 << logT) * g > cap) g >>= 1;
@@ -33,7 +33,7 @@ int bw_pass_max_G(int logT, int limb_bits) {
     return g;
 }
 
-int bw_reg2_lds_line(int logT) { return (1 << logT) + ((1 << logT) >> t2_r1log(logT)) + 1; }
+int bw_reg2_lds_line(int logT, bool cols) { return t2_lds_line(logT, cols); }
 
 hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_t stream) {
     PassKernel k = nullptr;
@@ -60,8 +60,8 @@ hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_
     if (!k) return hipErrorInvalidValue;
     // LDS element: 8 bytes, except the register-staged 32-bit path (centered int32)
     const size_t vbytes = (reg2 && L.limb_bits == 32) ? 4 : 8;
-    const size_t lds = (size_t)L.G * (reg2 ? bw_reg2_lds_line(L.logT) : lds_line(L.logT)) * vbytes;
-    if (L.G > bw_pass_max_G(L.logT, L.limb_bits) || (L.G & (L.G - 1))) return hipErrorInvalidValue;
+    const size_t lds = (size_t)L.G * (reg2 ? bw_reg2_lds_line(L.logT, prm.S > 1) : lds_line(L.logT)) * vbytes;
+    if (L.G > bw_pass_max_G(L.logT, L.limb_bits, prm.S > 1) || (L.G & (L.G - 1))) return hipErrorInvalidValue;
     if (prm.bdesc && prm.ingress && !reg2) return hipErrorInvalidValue;   // stacked ingress: reg2 only
     {
         // Allow the largest tile this kernel is ever launched with, once per kernel.
@@ -69,8 +69,8 @@ hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_
         static std::unordered_set<const void*> done;
         std::lock_guard<std::mutex> lock(mu);
         if (!done.count(reinterpret_cast<const void*>(k))) {
-            const size_t lds_max = (size_t)bw_pass_max_G(L.logT, L.limb_bits) *
-                                   (reg2 ? bw_reg2_lds_line(L.logT) : lds_line(L.logT)) * vbytes;
+            const size_t lds_max = (size_t)bw_pass_max_G(L.logT, L.limb_bits, prm.S > 1) *
+                                   (reg2 ? bw_reg2_lds_line(L.logT, prm.S > 1) : lds_line(L.logT)) * vbytes;
             hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max);
             if (e != hipSuccess) return e;
@@ -79,7 +79,7 @@ hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_
     }
     dim3 grid((unsigned)L.tiles, (unsigned)L.nprimes, (unsigned)(L.mode == BW_MODE_FWD ? L.nops : 1));
     // register-staged kernels run G * R1 threads (one stage-0 group each)
-    const unsigned nt = reg2 ? (unsigned)(L.G << t2_r1log(L.logT)) : 256u;
+    const unsigned nt = reg2 ? (unsigned)(L.G << t2_r1log(L.logT, prm.S > 1)) : 256u;
     if (nt > 512 || nt == 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k, grid, dim3(nt), lds, stream, prm);
     return hipGetLastError();
