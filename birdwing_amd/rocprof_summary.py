@@ -2,11 +2,11 @@
 
   python -m birdwing_amd.rocprof_summary stats DIR        kernel-trace --stats table (markdown)
   python -m birdwing_amd.rocprof_summary pmc DIR          per-kernel average of each PMC counter
-  python -m birdwing_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR KERNEL_REGEX CALIB_MIB
-        HBM bytes per dispatch of the kernels matching KERNEL_REGEX, from a FETCH_SIZE
-        run and a WRITE_SIZE run (KiB per dispatch), each scaled by what the same
-        counter reads for the calibration copy kernel (CALIB_MIB MiB read and written)
-        recorded in the same runs (JSON).
+  python -m birdwing_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR CALIB_MIB LABEL=REGEX ...
+        HBM bytes per dispatch of each labelled kernel (kernels matching REGEX), from a
+        FETCH_SIZE run and a WRITE_SIZE run (KiB per dispatch), each scaled by what the
+        same counter reads for the calibration kernel (one elementwise kernel reading and
+        writing CALIB_MIB MiB) recorded in the same runs (JSON).
 """
 import csv
 import glob
@@ -84,22 +84,23 @@ def _match(t, regex, counter):
     return tot / cnt, cnt
 
 
-def traffic(fetch_dir, write_dir, regex, calib_mib):
+def traffic(fetch_dir, write_dir, calib_mib, pairs):
+    """pairs: [(label, regex)]; HBM bytes per dispatch of each labelled kernel."""
     tf, tw = pmc_table(fetch_dir), pmc_table(write_dir)
-    calib = r"bitwise_not|elementwise"
+    calib = r"bitwise_not"
     cf, _ = _match(tf, calib, "FETCH_SIZE")
     cw, _ = _match(tw, calib, "WRITE_SIZE")
-    f, nf = _match(tf, regex, "FETCH_SIZE")
-    w, nw = _match(tw, regex, "WRITE_SIZE")
     kib = float(calib_mib) * 1024.0
     ff, fw = kib / cf, kib / cw           # true KiB per counted KiB
-    return {
-        "kernel_regex": regex,
-        "fetch_kib_per_dispatch": f, "write_kib_per_dispatch": w, "dispatches": [nf, nw],
-        "calibration": {"copy_mib": calib_mib, "fetch_kib": cf, "write_kib": cw,
-                        "fetch_factor": ff, "write_factor": fw},
-        "hbm_bytes_per_launch": (f * ff + w * fw) * 1024.0,
-    }
+    out = {"calibration": {"kernel": "one elementwise kernel reading and writing %g MiB" % calib_mib,
+                           "fetch_kib": cf, "write_kib": cw, "fetch_factor": ff, "write_factor": fw},
+           "kernels": {}}
+    for label, regex in pairs:
+        f, nf = _match(tf, regex, "FETCH_SIZE")
+        w, nw = _match(tw, regex, "WRITE_SIZE")
+        out["kernels"][label] = {"regex": regex, "fetch_kib_per_dispatch": f, "write_kib_per_dispatch": w,
+                                 "dispatches": [nf, nw], "hbm_bytes_per_launch": (f * ff + w * fw) * 1024.0}
+    return out
 
 
 def main(argv):
@@ -108,7 +109,8 @@ def main(argv):
     elif len(argv) >= 2 and argv[0] == "pmc":
         print(pmc(argv[1]))
     elif len(argv) >= 5 and argv[0] == "traffic":
-        print(json.dumps(traffic(argv[1], argv[2], argv[3], float(argv[4])), indent=1))
+        pairs = [a.rsplit("=", 1) for a in argv[4:]]   # labels may hold "=", regexes do not
+        print(json.dumps(traffic(argv[1], argv[2], float(argv[3]), pairs), indent=1))
     else:
         print(__doc__)
         return 2
