@@ -68,7 +68,9 @@ def launch_names(plan, schedule):
 
 
 def launch_bytes(plan, bits, names):
-    """Algorithmic bytes per launch (staged one-pass model).
+    """Algorithmic bytes per launch (staged one-pass model) for what the launch
+    processes (k primes of N_exec residue words; 4-byte words for 32-bit limbs
+    and for 64-bit digits, 8-byte words on the 64-bit-prime path).
 
     fwd pass 1 (ingress):  read A,B limbs, write 2k spectra      = S1
     fwd pass j > 1:        read + write 2k planes
@@ -76,21 +78,22 @@ def launch_bytes(plan, bits, names):
     inv pass j > 1:        read + write k planes
     inv pass 1:            read + write k planes                   = S3
     garner_carry:          read k planes, write L_A+L_B limbs      = S4+S5 without the
-                           128/160-bit coefficient round trip the fusion removes
+                           wide-coefficient round trip the fusion removes
     """
-    w = bits // 8
+    wl = bits // 8
+    w = 4 if (bits == 32 or plan.digit_bits == 64) else 8
     N = plan.N_exec
     L = plan.L_A + plan.L_B
     P = len(plan.passes)
     out = []
     for name, planes in names:
         if name.startswith("garner"):
-            out.append(plan.k * N * w + L * w)
+            out.append(plan.k * N * w + L * wl)
         elif name.startswith("fused"):
-            out.append((L * w + planes * N * w) if P == 1 else 3 * planes * N * w)
+            out.append((L * wl + planes * N * w) if P == 1 else 3 * planes * N * w)
         elif name.startswith("fwd_pass1"):
             nop = 2 if name.endswith("AB") else 1
-            raw = L * w if nop == 2 else (plan.L_A if name.endswith("A") else plan.L_B) * w
+            raw = L * wl if nop == 2 else (plan.L_A if name.endswith("A") else plan.L_B) * wl
             out.append(raw + planes * N * w)
         else:
             out.append(2 * planes * N * w)
@@ -98,10 +101,15 @@ def launch_bytes(plan, bits, names):
 
 
 def model_bytes(plan, bits):
-    """Whole-product bytes of the staged model: (L_A+L_B)(w_in+w_out) + 8kNw_r + 2Nw_c."""
+    """Whole-product bytes of the staged model of the reference method (fixed,
+    whatever this library runs): three primes on digits of the limb width,
+    N = next_pow2(L_A + L_B - 1), (L_A+L_B)(w_in+w_out) + 8kNw_r + 2Nw_c with
+    (w_in, w_r, w_c, w_out) = (4, 4, 16, 4) B for 32-bit limbs, (8, 8, 20, 8) B for 64-bit."""
+    import birdwing_amd as bw
     w = bits // 8
     wc = 16 if bits == 32 else 20
-    return (plan.L_A + plan.L_B) * 2 * w + 8 * plan.k * plan.N * w + 2 * plan.N * wc
+    N = bw.padded_ntt_size(plan.L_A, plan.L_B)
+    return (plan.L_A + plan.L_B) * 2 * w + 8 * 3 * N * w + 2 * N * wc
 
 
 def load_traffic(workload, kernel):

@@ -22,7 +22,7 @@ import os
 import numpy as np
 
 __all__ = [
-    "lib", "Plan", "multiply", "BWError", "SET_DEFAULT", "SET_U32_REF", "SET_U32_WIDE", "SET_U64_REF",
+    "lib", "Plan", "multiply", "BWError", "SET_DEFAULT", "SET_U32_REF", "SET_U32_WIDE", "SET_U64_REF", "SET_D64",
     "padded_ntt_size", "multiply_size_supported", "ntt_size_supported", "crt_bound_satisfied",
     "max_supported_logN", "max_supported_limb_count", "max_limb_count_by_crt", "default_prime_set",
     "prime_set_info", "crt_combine", "carry128", "carry160", "zero_pad", "zero_pad_u64", "pointwise",
@@ -32,7 +32,7 @@ __all__ = [
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbirdwing_hip.so")
 
-SET_DEFAULT, SET_U32_REF, SET_U32_WIDE, SET_U64_REF, SET_HYBRID = -1, 0, 1, 2, 3
+SET_DEFAULT, SET_U32_REF, SET_U32_WIDE, SET_U64_REF, SET_HYBRID, SET_D64 = -1, 0, 1, 2, 3, 4
 BW_OK, BW_ERR_ARG, BW_ERR_UNSUPPORTED_SIZE, BW_ERR_HIP, BW_ERR_NOMEM, BW_ERR_STATE = 0, -1, -2, -3, -4, -5
 
 
@@ -44,9 +44,10 @@ class BWError(RuntimeError):
 
 class PlanInfo(C.Structure):
     _fields_ = [("limb_bits", C.c_int), ("prime_set", C.c_int), ("k", C.c_int),
-                ("primes", C.c_uint64 * 3), ("L_A", C.c_uint64), ("L_B", C.c_uint64),
+                ("primes", C.c_uint64 * 5), ("L_A", C.c_uint64), ("L_B", C.c_uint64),
                 ("N", C.c_uint64), ("N_exec", C.c_uint64), ("npass", C.c_int),
-                ("pass_log", C.c_int * 4), ("nlaunch", C.c_int), ("device_bytes", C.c_uint64)]
+                ("pass_log", C.c_int * 4), ("nlaunch", C.c_int), ("device_bytes", C.c_uint64),
+                ("digit_bits", C.c_int)]
 
 
 class BatchInfo(C.Structure):
@@ -69,6 +70,7 @@ _vp, _u64, _i, _sz, _cp = C.c_void_p, C.c_uint64, C.c_int, C.c_size_t, C.c_char_
 _pi, _pu64 = C.POINTER(C.c_int), C.POINTER(C.c_uint64)
 # name -> (restype, argtypes); every symbol declared in include/birdwing.h
 SIGNATURES = {
+    "bw_abi_version": (_i, []),
     "bw_last_error": (_cp, []),
     "bw_version": (_cp, []),
     "bw_init": (_i, [_i]),
@@ -180,8 +182,8 @@ def default_prime_set(limb_bits, N):
 
 def prime_set_info(s):
     k = C.c_int()
-    P = (C.c_uint64 * 3)()
-    R = (C.c_uint64 * 3)()
+    P = (C.c_uint64 * 5)()
+    R = (C.c_uint64 * 5)()
     _check(lib().bw_prime_set_info(s, C.byref(k), P, R))
     return [int(P[i]) for i in range(k.value)], [int(R[i]) for i in range(k.value)]
 
@@ -235,6 +237,7 @@ class Plan:
         self.primes = [int(info.primes[i]) for i in range(self.k)]
         self.passes = [int(info.pass_log[i]) for i in range(info.npass)]
         self.prime_set = int(info.prime_set)
+        self.digit_bits = int(info.digit_bits)
 
     def close(self):
         if self._p:

@@ -34,14 +34,16 @@ using namespace bw;
 
 namespace bw { const char* last_error(); }
 extern "C" const char* bw_last_error(void) { return bw::last_error(); }
-extern "C" const char* bw_version(void) { return "birdwing-mi355x 0.1 (gfx950)"; }
+extern "C" const char* bw_version(void) { return "birdwing-mi355x 0.2 (gfx950)"; }
+extern "C" int bw_abi_version(void) { return BW_ABI_VERSION; }
 
 extern "C" int bw_init(int device) {
     HIP_TRY(hipSetDevice(device));
     DevTables* t;
     int rc = get_tables(SET_U32_REF, &t);
-    if (rc) return rc;
-    return get_tables(SET_U32_WIDE, &t);
+    if (!rc) rc = get_tables(SET_U32_WIDE, &t);
+    if (!rc) rc = get_tables(SET_D64, &t);
+    return rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -110,7 +112,12 @@ extern "C" int bw_multiply_size_supported(int set, int in_bits, uint64_t L_A, ui
 #define BW_MAX_LAUNCH 32
 
 struct bw_plan {
-    int limb_bits = 32;
+    int limb_bits = 32;             // caller's limb width
+    int arith_bits = 32;            // 32: FP64 kernels, 4-byte residues; 64: integer kernels
+    int ingress = 1;                // 1: a limb is a digit; 2: 64-bit digits read as 32-bit limb pairs
+    int64_t in_A = 0, in_B = 0;     // operand lengths in the ingress's raw units
+    uint64_t n_words = 0;           // output words of the carry (64-bit words for 64-bit digits)
+    int half_last = 0;              // last output word is half a word (odd 32-bit-limb product)
     int set_id = 0;
     const PrimeSet* set = nullptr;
     DevTables* tabs = nullptr;
@@ -169,20 +176,27 @@ extern "C" int bw_plan_create_ex(bw_plan** out, int limb_bits, uint64_t L_A, uin
     *out = nullptr;
     if (limb_bits != 32 && limb_bits != 64) return fail(BW_ERR_ARG, "limb_bits must be 32 or 64");
     if (L_A == 0 || L_B == 0) return fail(BW_ERR_UNSUPPORTED_SIZE, "operand limb count must be positive");
-    const uint64_t N = padded_ntt_size(L_A, L_B);
-    if (set_id < 0) set_id = default_set(limb_bits, N);
+    if (set_id < 0) set_id = auto_set(limb_bits, L_A, L_B);
     const PrimeSet* s = prime_set(set_id);
     if (!s) return fail(BW_ERR_ARG, "unknown prime set %d", set_id);
-    if (s->limb_bits != limb_bits || s->k != 3)
+    const bool d64 = s->digit_bits == 64;
+    if (d64 ? s->k != 5 : (s->limb_bits != limb_bits || s->k != 3))
         return fail(BW_ERR_ARG, "prime set %d does not serve %d-bit limbs", set_id, limb_bits);
     std::string why;
     if (!multiply_size_supported(*s, limb_bits, L_A, L_B, &why))
         return fail(BW_ERR_UNSUPPORTED_SIZE, "unsupported multiply: %s", why.c_str());
+    const uint64_t N = padded_size_for(*s, limb_bits, L_A, L_B);
     const uint64_t N_exec = N < 2 ? 2 : N;
     const int logN = valuation2(N_exec);
 
     bw_plan* p = new bw_plan();
     p->limb_bits = limb_bits;
+    p->arith_bits = s->limb_bits;
+    p->ingress = d64 ? 2 : 1;
+    p->in_A = (int64_t)(d64 ? L_A * (limb_bits / 32) : L_A);
+    p->in_B = (int64_t)(d64 ? L_B * (limb_bits / 32) : L_B);
+    p->n_words = d64 && limb_bits == 32 ? (L_A + L_B + 1) / 2 : L_A + L_B;
+    p->half_last = d64 && limb_bits == 32 && ((L_A + L_B) & 1);
     p->set_id = set_id;
     p->set = s;
     p->L_A = L_A;
@@ -190,13 +204,14 @@ extern "C" int bw_plan_create_ex(bw_plan** out, int limb_bits, uint64_t L_A, uin
     p->N = N;
     p->N_exec = N_exec;
     p->logN = logN;
-    p->word = limb_bits == 64 ? 8 : 4;
+    p->word = s->limb_bits == 64 ? 8 : 4;
     if (pass_log && npass > 0) {
         int sum = 0;
         for (int i = 0; i < npass; i++) {
-            if (pass_log[i] < 1 || pass_log[i] > BW_MAX_PASS_LOG) {
+            const int maxlog = d64 ? BW_REG2_MAX_LOG : BW_MAX_PASS_LOG;   // 64-bit digits: register-staged passes
+            if (pass_log[i] < 1 || pass_log[i] > maxlog) {
                 delete p;
-                return fail(BW_ERR_ARG, "pass_log[%d]=%d outside [1, %d]", i, pass_log[i], BW_MAX_PASS_LOG);
+                return fail(BW_ERR_ARG, "pass_log[%d]=%d outside [1, %d]", i, pass_log[i], maxlog);
             }
             sum += pass_log[i];
             p->passes.push_back(pass_log[i]);
@@ -207,7 +222,7 @@ extern "C" int bw_plan_create_ex(bw_plan** out, int limb_bits, uint64_t L_A, uin
         }
     } else {
         TuneChoice c;
-        if (tune_lookup(TuneKey{limb_bits, logN, 0}, &c)) {   // autotuned split for this size
+        if (tune_lookup(TuneKey{set_id, logN, 0}, &c)) {   // autotuned split for this size
             p->passes = c.passes;
             p->gshift = c.gshift;
         } else {
@@ -225,10 +240,10 @@ extern "C" int bw_plan_create_ex(bw_plan** out, int limb_bits, uint64_t L_A, uin
     std::vector<double> nf;
     ninv_pairs(*s, N_exec, n64, nf);
     if ((rc = plan_alloc(&p->ninv, 16 * s->k, p))) { plan_free(p); return rc; }
-    hipError_t e = hipMemcpy(p->ninv, limb_bits == 64 ? (const void*)n64.data() : (const void*)nf.data(),
+    hipError_t e = hipMemcpy(p->ninv, s->limb_bits == 64 ? (const void*)n64.data() : (const void*)nf.data(),
                              16 * s->k, hipMemcpyHostToDevice);
     if (e != hipSuccess) { plan_free(p); return fail(BW_ERR_HIP, "hipMemcpy: %s", hipGetErrorString(e)); }
-    p->ntiles = bw_carry_tiles((int64_t)(L_A + L_B));
+    p->ntiles = bw_carry_tiles((int64_t)p->n_words);
     if ((rc = plan_alloc((void**)&p->flags, 8 * (size_t)p->ntiles, p)) ||
         (rc = plan_alloc((void**)&p->counter, 2 * sizeof(int), p))) {
         plan_free(p);
@@ -251,7 +266,8 @@ extern "C" void bw_plan_destroy(bw_plan* p) { plan_free(p); }
 
 static Xform plan_xform(const bw_plan* p) {
     Xform x;
-    x.limb_bits = p->limb_bits;
+    x.limb_bits = p->arith_bits;
+    x.ingress = p->ingress;
     x.set = p->set;
     x.tabs = p->tabs;
     x.logN = p->logN;
@@ -283,6 +299,7 @@ extern "C" int bw_plan_get_info(const bw_plan* p, bw_plan_info* info) {
     for (int i = 0; i < info->npass && i < 4; i++) info->pass_log[i] = p->passes[i];
     info->nlaunch = count_launches(p);
     info->device_bytes = p->dev_bytes;
+    info->digit_bits = p->ingress == 2 ? 64 : p->limb_bits;
     return BW_OK;
 }
 
@@ -314,7 +331,7 @@ struct Marker : LaunchMarker {
 };
 
 static int run_plan_transforms(bw_plan* p, const void* d_a, const void* d_b, hipStream_t st, Marker& mk) {
-    return run_transforms(plan_xform(p), d_a, d_b, (int64_t)p->L_A, (int64_t)p->L_B, st, &mk);
+    return run_transforms(plan_xform(p), d_a, d_b, p->in_A, p->in_B, st, &mk);
 }
 
 static int check_plan(const bw_plan* p) {
@@ -333,11 +350,13 @@ extern "C" int bw_execute(bw_plan* p, const void* d_a, const void* d_b, void* d_
     if ((rc = run_plan_transforms(p, d_a, d_b, st, mk))) return rc;
     CarryParams cp;
     std::memset(&cp, 0, sizeof(cp));
-    cp.src = p->limb_bits == 64 ? BW_SRC_GARNER_U64 : BW_SRC_GARNER_U32;
+    cp.src = p->ingress == 2 ? BW_SRC_GARNER_D64 : (p->arith_bits == 64 ? BW_SRC_GARNER_U64 : BW_SRC_GARNER_U32);
+    cp.gd = p->tabs->gd;
+    cp.half_last = p->half_last;
     cp.a0 = p->bufA;
     cp.plane = (int64_t)p->N_exec;
     cp.N = (int64_t)p->N_exec;
-    cp.n_out = (int64_t)(p->L_A + p->L_B);
+    cp.n_out = (int64_t)p->n_words;
     cp.out = d_c;
     cp.g = p->tabs->garner;
     for (int i = 0; i < 3; i++) cp.fp[i] = p->tabs->fp[i];
@@ -355,6 +374,9 @@ extern "C" int bw_execute_crt(bw_plan* p, const void* d_a, const void* d_b, uint
                               uint32_t* d_c2, void* stream) {
     int rc = check_plan(p);
     if (rc) return rc;
+    if (p->ingress == 2)
+        return fail(BW_ERR_STATE, "plan runs 64-bit digits under five primes; CRT coefficients of the "
+                                  "reference pipelines need a plan with an explicit three-prime set");
     if (!d_a || !d_b || !d_c0 || !d_c1 || (p->limb_bits == 64 && !d_c2)) return fail(BW_ERR_ARG, "NULL device buffer");
     hipStream_t st = (hipStream_t)stream;
     Marker mk(p, st);
@@ -415,6 +437,7 @@ extern "C" int bw_plan_tune(bw_plan* p, const void* d_a, const void* d_b, void* 
     for (const auto& c : cands) {
         bool reg2 = true;
         for (int t : c.passes) reg2 = reg2 && t <= BW_REG2_MAX_LOG;
+        if (!reg2 && p->ingress == 2) continue;
         if (!reg2 && c.passes != p->passes) continue;
         Xform x = plan_xform(p);
         x.passes = c.passes;
@@ -422,7 +445,7 @@ extern "C" int bw_plan_tune(bw_plan* p, const void* d_a, const void* d_b, void* 
         float ms_min = -1.0f;
         for (int r = 0; r < reps + 1 && !rc; r++) {
             if (hipEventRecord(e0, st) != hipSuccess) { rc = fail(BW_ERR_HIP, "hipEventRecord"); break; }
-            if ((rc = run_transforms(x, d_a, d_b, (int64_t)p->L_A, (int64_t)p->L_B, st, nullptr))) break;
+            if ((rc = run_transforms(x, d_a, d_b, p->in_A, p->in_B, st, nullptr))) break;
             if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
                 rc = fail(BW_ERR_HIP, "tuning event");
                 break;
@@ -443,7 +466,7 @@ extern "C" int bw_plan_tune(bw_plan* p, const void* d_a, const void* d_b, void* 
     if (best.ms >= 0) {
         p->passes = best.passes;
         p->gshift = best.gshift;
-        tune_store(TuneKey{p->limb_bits, p->logN, 0}, best);
+        tune_store(TuneKey{p->set_id, p->logN, 0}, best);
     }
     return BW_OK;
 }
@@ -523,9 +546,9 @@ extern "C" int bw_forward(bw_plan* p, const void* d_raw, uint64_t L, void* d_spe
         PassParams prm = base_params(x, g);
         prm.nops = 1;
         if (j == 0) {
-            prm.ingress = 1;
+            prm.ingress = p->ingress;
             prm.in[0] = d_raw;
-            prm.in_len[0] = (int64_t)L;
+            prm.in_len[0] = (int64_t)(p->ingress == 2 ? L * (p->limb_bits / 32) : L);
         } else {
             prm.in[0] = d_spec;
         }
@@ -549,7 +572,7 @@ extern "C" int bw_inverse(bw_plan* p, void* d_spec, void* stream) {
         prm.out[0] = d_spec;
         LAUNCH_TRY(launch_pass(x, BW_MODE_INV, g, prm, 1, st));
     }
-    LAUNCH_TRY(bw_launch_scale(p->limb_bits, d_spec, (int64_t)p->N_exec, (int64_t)p->N_exec, p->set->k,
+    LAUNCH_TRY(bw_launch_scale(p->arith_bits, d_spec, (int64_t)p->N_exec, (int64_t)p->N_exec, p->set->k,
                                p->tabs->t.consts, p->ninv, st));
     return BW_OK;
 }

@@ -44,6 +44,13 @@ static int64_t rows_tile(int logN, int limb_bits, int gshift) {
     return g;
 }
 
+// Carry words of product i: 64-bit words for 64-bit digits (half a word
+// more than needed when an odd number of 32-bit limbs), else its limbs.
+static uint64_t carry_words(const BatchGroup& g, const BatchLayout& L, int i) {
+    const uint64_t n = L.LA[i] + L.LB[i];
+    return (g.ingress == 2 && L.bits[i] == 32) ? (n + 1) / 2 : n;
+}
+
 static void group_geometry(BatchGroup& g) {
     const int64_t m = (int64_t)g.members.size();
     g.B = m;
@@ -64,17 +71,17 @@ int batch_layout(int n, const int* bits, const uint64_t* LA, const uint64_t* LB,
     L.offA.resize(n);
     L.offB.resize(n);
     L.offC.resize(n);
-    std::map<std::pair<int, int>, int> gidx;   // (bits, logN) -> group
+    std::map<std::pair<int, int>, int> gidx;   // (prime set, logN) -> group
     for (int i = 0; i < n; i++) {
         const int b = bits[i];
         if (b != 32 && b != 64) return fail(BW_ERR_ARG, "product %d: limb_bits must be 32 or 64", i);
         if (LA[i] == 0 || LB[i] == 0) return fail(BW_ERR_UNSUPPORTED_SIZE, "product %d: empty operand", i);
-        const uint64_t N = padded_ntt_size(LA[i], LB[i]);
-        const int set_id = default_set(b, N);
+        const int set_id = auto_set(b, LA[i], LB[i]);
         const PrimeSet* s = prime_set(set_id);
         std::string why;
         if (!s || !multiply_size_supported(*s, b, LA[i], LB[i], &why))
             return fail(BW_ERR_UNSUPPORTED_SIZE, "product %d: unsupported multiply: %s", i, why.c_str());
+        const uint64_t N = padded_size_for(*s, b, LA[i], LB[i]);
         const size_t w = (size_t)b / 8;
         L.offA[i] = L.bytesA;
         L.offB[i] = L.bytesB;
@@ -84,11 +91,12 @@ int batch_layout(int n, const int* bits, const uint64_t* LA, const uint64_t* LB,
         L.bytesC = align_up(L.bytesC + (LA[i] + LB[i]) * w, 256);
         const uint64_t Nx = N < 2 ? 2 : N;
         const int logN = valuation2(Nx);
-        auto key = std::make_pair(b, logN);
+        auto key = std::make_pair(set_id, logN);
         auto it = gidx.find(key);
         if (it == gidx.end()) {
             BatchGroup g;
-            g.limb_bits = b;
+            g.limb_bits = s->limb_bits;
+            g.ingress = s->digit_bits == 64 ? 2 : 1;
             g.logN = logN;
             g.N = (int64_t)Nx;
             g.set_id = set_id;
@@ -103,7 +111,7 @@ int batch_layout(int n, const int* bits, const uint64_t* LA, const uint64_t* LB,
     // pass split and tile cap: tuned choice if known, else the default split
     for (auto& g : L.groups) {
         TuneChoice c;
-        if (tune_lookup(TuneKey{g.limb_bits, g.logN, log2_bucket((int64_t)g.members.size())}, &c)) {
+        if (tune_lookup(TuneKey{g.set_id, g.logN, log2_bucket((int64_t)g.members.size())}, &c)) {
             g.passes = c.passes;
             g.gshift = c.gshift;
         } else {
@@ -126,7 +134,7 @@ int batch_layout(int n, const int* bits, const uint64_t* LA, const uint64_t* LB,
         BatchCarry& c = L.carries[cidx[g.set_id]];
         for (int i : g.members) {
             c.njobs++;
-            c.ntiles += bw_carry_tiles((int64_t)(L.LA[i] + L.LB[i]));
+            c.ntiles += bw_carry_tiles((int64_t)carry_words(g, L, i));
         }
     }
     // workspace: residue planes first, then the descriptor block, then flags
@@ -142,7 +150,7 @@ int batch_layout(int n, const int* bits, const uint64_t* LA, const uint64_t* LB,
     L.off_desc = off;
     for (auto& g : L.groups) {
         g.off_ninv = off;
-        off = align_up(off + 16 * 3, 64);
+        off = align_up(off + 16 * (size_t)prime_set(g.set_id)->k, 64);   // (N^-1, aux) pair per prime
         g.off_ing = off;
         off = align_up(off + sizeof(BwIngress) * (size_t)g.cap, 64);
     }
@@ -178,15 +186,17 @@ void batch_descriptors(const BatchLayout& L, void* ws, std::vector<unsigned char
         std::memcpy(at(g.off_ninv), g.limb_bits == 64 ? (const void*)n64.data() : (const void*)nf.data(),
                     16 * (size_t)s.k);
         BwIngress* ing = reinterpret_cast<BwIngress*>(at(g.off_ing));
-        const size_t w = (size_t)g.limb_bits / 8;
         for (int64_t slot = 0; slot < g.B; slot++) {
             BwIngress d = {0, 0, 0, 0};
             if (slot < (int64_t)g.members.size()) {
                 const int i = g.members[slot];
-                d.a_off = (int64_t)(L.offA[i] / w);
-                d.b_off = (int64_t)(L.offB[i] / w);
-                d.a_len = (int64_t)L.LA[i];
-                d.b_len = (int64_t)L.LB[i];
+                // raw units: 32-bit limbs for 64-bit digits, else the product's limbs
+                const size_t u = g.ingress == 2 ? 4 : (size_t)L.bits[i] / 8;
+                const uint64_t per = g.ingress == 2 ? (uint64_t)L.bits[i] / 32 : 1;
+                d.a_off = (int64_t)(L.offA[i] / u);
+                d.b_off = (int64_t)(L.offB[i] / u);
+                d.a_len = (int64_t)(L.LA[i] * per);
+                d.b_len = (int64_t)(L.LB[i] * per);
             }
             ing[slot] = d;
         }
@@ -205,9 +215,11 @@ void batch_descriptors(const BatchLayout& L, void* ws, std::vector<unsigned char
                 J.res = dws + g.off_bufA + slot * (size_t)g.N * w;
                 J.plane = g.B * g.N;
                 J.N = g.N;
-                J.n_out = (int64_t)(L.LA[i] + L.LB[i]);
-                J.out_off = (int64_t)(L.offC[i] / w);
+                J.n_out = (int64_t)carry_words(g, L, i);
+                // output words: 64-bit for 64-bit digits, else the product's limbs
+                J.out_off = (int64_t)(L.offC[i] / (g.ingress == 2 ? 8 : (size_t)L.bits[i] / 8));
                 J.tile0 = t;
+                J.half_last = g.ingress == 2 && L.bits[i] == 32 && ((L.LA[i] + L.LB[i]) & 1);
                 const int64_t nt = bw_carry_tiles(J.n_out);
                 for (int64_t q = 0; q < nt; q++) tile_job[t + q] = j;
                 t += nt;
@@ -231,6 +243,7 @@ static Xform group_xform(const BatchGroup& g, void* ws) {
     x.bufB = d + g.off_bufB;
     x.ninv = d + g.off_ninv;
     x.schedule = BW_SCHED_PASS_MAJOR;
+    x.ingress = g.ingress;
     x.bdesc = reinterpret_cast<const BwIngress*>(d + g.off_ing);
     return x;
 }
@@ -257,7 +270,9 @@ int batch_issue(const BatchLayout& L, void* ws, const void* dA, const void* dB, 
         LAUNCH_TRY(bw_launch_carry_reset(flags, c.ntiles, counter, st));
         CarryParams cp;
         std::memset(&cp, 0, sizeof(cp));
-        cp.src = c.limb_bits == 64 ? BW_SRC_GARNER_U64 : BW_SRC_GARNER_U32;
+        cp.src = prime_set(c.set_id)->digit_bits == 64 ? BW_SRC_GARNER_D64
+                 : (c.limb_bits == 64 ? BW_SRC_GARNER_U64 : BW_SRC_GARNER_U32);
+        cp.gd = tabs->gd;
         cp.out = dC;
         cp.g = tabs->garner;
         for (int i = 0; i < 3; i++) cp.fp[i] = tabs->fp[i];
@@ -520,7 +535,7 @@ extern "C" int bw_batch_tune(bw_batch* b, const void* d_a, const void* d_b, void
             g.passes = best.passes;
             g.gshift = best.gshift;
             group_geometry(g);
-            tune_store(TuneKey{g.limb_bits, g.logN, log2_bucket(m)}, best);
+            tune_store(TuneKey{g.set_id, g.logN, log2_bucket(m)}, best);
         }
     }
     (void)hipEventDestroy(e0);

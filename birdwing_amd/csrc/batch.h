@@ -17,7 +17,8 @@
 namespace bw {
 
 struct BatchGroup {
-    int limb_bits = 32;
+    int limb_bits = 32;             // arithmetic width of the set (32: FP64 kernels)
+    int ingress = 1;                // 2: 64-bit digits read as 32-bit limb pairs
     int set_id = 0;
     int logN = 0;
     int64_t N = 0;
@@ -54,7 +55,7 @@ struct BatchLayout {
 
 // Tuned (pass split, tile cap) per (limb width, log2 N, log2 B bucket).
 struct TuneKey {
-    int bits, logN, logB;
+    int bits, logN, logB;           // bits: the prime set id
     bool operator<(const TuneKey& o) const {
         return bits != o.bits ? bits < o.bits : logN != o.logN ? logN < o.logN : logB < o.logB;
     }

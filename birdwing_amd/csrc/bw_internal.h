@@ -5,7 +5,7 @@
 #include <stddef.h>
 #include "arith.h"
 
-#define BW_MAX_PRIMES 3
+#define BW_MAX_PRIMES 5
 #define BW_STAGE_LOG 13          // stage twiddle table: omega_{2^13}^e, e < 2^13
 #define BW_TW_LO_LOG 12          // pass twiddles: 2-level table split at 2^12
 #define BW_MAX_PASS_LOG 13       // largest sub-transform held in one workgroup
@@ -15,6 +15,7 @@ struct F64PrimeDev {
     F64Prime P;
     double w32[16][2];    // omega_32^e, e < 16: (w, w/p)
     double iw32[16][2];   // omega_32^-e
+    double c32[2];        // 2^32 mod p as (w, w/p): 64-bit digits enter as hi * 2^32 + lo
 };
 
 // Per-prime constants of the u64 digit path (Shoup pairs (w, floor(w 2^64/p))).
@@ -45,6 +46,14 @@ struct GarnerDev {
     uint64_t inv1_s[2], inv2_s[2], p0mod2_s[2];
     uint64_t p0;           // p0
     uint64_t p01_lo, p01_hi;   // p0*p1 as 128-bit
+};
+
+// Garner constants of the five-prime 64-bit-digit set (FP64 arithmetic):
+// inv[j][i] = p_i^-1 mod p_j (i < j) as (w, w/p_j) pairs.
+struct GarnerD64Dev {
+    F64Prime fp[5];
+    double inv[5][4][2];
+    uint32_t p[5];
 };
 
 // One pass of the multi-pass transform.  Element (o, t, s) of the N-array lives

@@ -22,6 +22,7 @@ enum {
     BW_SRC_HILO128 = 2,      // (hi, lo) u64 arrays -> 32-bit limbs
     BW_SRC_U160 = 3,         // (lo, mid, hi32) arrays -> 64-bit limbs
     BW_SRC_HILO128_L64 = 4,  // (hi, lo) u64 arrays -> 64-bit limbs
+    BW_SRC_GARNER_D64 = 5,   // 5 canonical u32 residue planes of 64-bit digits -> 64-bit limbs
 };
 
 // Batched carry: one launch covers many products; product j owns tiles
@@ -34,6 +35,7 @@ struct BwCarryJob {
     int64_t n_out;
     int64_t out_off;      // limbs from CarryParams::out
     int64_t tile0;
+    int64_t half_last;    // 1: the last limb is written as its low 32 bits only
 };
 
 struct CarryParams {
@@ -47,6 +49,9 @@ struct CarryParams {
     void* out;
     GarnerDev g;
     F64Prime fp[3];       // FP64 constants of the three primes (u32 path)
+    GarnerD64Dev gd;      // five-prime constants (64-bit digits)
+    int half_last;        // 1: the last output limb is written as its low 32 bits only
+                          // (a 32-bit-limb product of odd length, computed in 64-bit limbs)
     unsigned long long* flags;   // one per tile
     int* counter;         // tile-id allocator, returns to 0 after the launch
     uint32_t epoch;       // tags this launch's flags

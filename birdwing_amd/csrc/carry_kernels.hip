@@ -46,6 +46,36 @@ __device__ __forceinline__ void garner_u64(uint64_t r0, uint64_t r1, uint64_t r2
     w2 = (uint64_t)((s1 >> 64) + (c >> 64));
 }
 
+// Five-prime Garner for 64-bit digits: mixed-radix digits
+//   k_j = (...((r_j - k_0) p_0^-1 - k_1) p_1^-1 ... - k_{j-1}) p_{j-1}^-1 mod p_j
+// in FP64 products, then x = k_0 + p_0 (k_1 + p_1 (k_2 + p_2 (k_3 + p_3 k_4)))
+// by Horner steps on 32-bit words; x < 2^154 -> three 64-bit words.
+__device__ __forceinline__ void garner_d64(const uint32_t* r, int64_t plane, int64_t n, const GarnerD64Dev& g,
+                                           uint64_t* w) {
+    uint32_t k[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        double t = (double)r[j * plane + n];
+#pragma unroll
+        for (int i = 0; i < j; i++) t = f64_mulmod(t - (double)k[i], g.inv[j][i][0], g.inv[j][i][1], g.fp[j]);
+        k[j] = f64_canon(t, g.fp[j]);
+    }
+    uint32_t x[5] = {k[4], 0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int j = 3; j >= 0; j--) {
+        uint64_t c = k[j];
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            c += (uint64_t)x[i] * g.p[j];
+            x[i] = (uint32_t)c;
+            c >>= 32;
+        }
+    }
+    w[0] = x[0] | ((uint64_t)x[1] << 32);
+    w[1] = x[2] | ((uint64_t)x[3] << 32);
+    w[2] = x[4];
+}
+
 // ---------------------------------------------------------------------------
 // Coefficient sources: words w_j(c_n) in limb width, zero outside [0, N).
 // The view names the coefficient arrays of the product a tile belongs to.
@@ -74,6 +104,14 @@ struct SrcGarnerU64 {
         if (n < 0 || n >= p.N) { w[0] = w[1] = w[2] = 0; return; }
         const uint64_t* r = reinterpret_cast<const uint64_t*>(p.a0);
         garner_u64(r[n], r[p.plane + n], r[2 * p.plane + n], c.g, w[0], w[1], w[2]);
+    }
+};
+struct SrcGarnerD64 {
+    using Word = uint64_t;
+    static constexpr int NW = 3;
+    __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
+        if (n < 0 || n >= p.N) { w[0] = w[1] = w[2] = 0; return; }
+        garner_d64(reinterpret_cast<const uint32_t*>(p.a0), p.plane, n, c.gd, w);
     }
 };
 struct SrcHiLo128 {     // a0 = hi, a1 = lo
@@ -142,6 +180,7 @@ __global__ __launch_bounds__(256) void garner_carry_kernel(CarryParams prm) {
     int64_t n_out = prm.n_out;
     Wd* out = reinterpret_cast<Wd*>(prm.out);
     int64_t ltile = tile;
+    bool half_last = prm.half_last != 0;
     if (prm.jobs) {
         const BwCarryJob J = prm.jobs[prm.tile_job[tile]];
         view.a0 = J.res;
@@ -150,6 +189,7 @@ __global__ __launch_bounds__(256) void garner_carry_kernel(CarryParams prm) {
         n_out = J.n_out;
         out += J.out_off;
         ltile = tile - J.tile0;
+        half_last = J.half_last != 0;
     }
     const int64_t wbase = ltile * BW_CARRY_TILE + (int64_t)wave * 64 * K;
 
@@ -287,7 +327,10 @@ __global__ __launch_bounds__(256) void garner_carry_kernel(CarryParams prm) {
         const uint64_t carries = s2 ^ P;           // carry into each lane
         const Wd limb = tval[i] + (Wd)((carries >> lane) & 1ULL);
         const int64_t n = wbase + i * 64 + lane;
-        if (n < n_out) out[n] = limb;
+        if (n < n_out) {
+            if (half_last && n == n_out - 1) reinterpret_cast<uint32_t*>(out + n)[0] = (uint32_t)limb;
+            else out[n] = limb;
+        }
         cin = cout;
     }
 }
@@ -316,6 +359,7 @@ hipError_t bw_launch_garner_carry(const CarryParams& p, hipStream_t stream) {
     case BW_SRC_HILO128: hipLaunchKernelGGL(garner_carry_kernel<SrcHiLo128>, grid, block, 0, stream, p); break;
     case BW_SRC_HILO128_L64: hipLaunchKernelGGL(garner_carry_kernel<SrcHiLo128L64>, grid, block, 0, stream, p); break;
     case BW_SRC_U160: hipLaunchKernelGGL(garner_carry_kernel<SrcU160>, grid, block, 0, stream, p); break;
+    case BW_SRC_GARNER_D64: hipLaunchKernelGGL(garner_carry_kernel<SrcGarnerD64>, grid, block, 0, stream, p); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

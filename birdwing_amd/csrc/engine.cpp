@@ -55,6 +55,7 @@ int get_tables(int set_id, DevTables** out) {
         HIP_TRY(hipMemcpy(*ptrs[i], srcs[i]->data(), srcs[i]->size(), hipMemcpyHostToDevice));
     }
     d->garner = garner_constants(*s);
+    if (s->k == 5) d->gd = garner_d64_constants(*s);
     for (int i = 0; i < 3; i++) f64_prime(i < s->k ? s->p[i] : 3, d->fp[i]);
     *out = d.get();
     g_tabs[key] = std::move(d);
@@ -146,7 +147,7 @@ int run_transforms(const Xform& x, const void* d_a, const void* d_b, int64_t la,
     if (P == 1) {
         PassPlan g = pass_geom(x, 0);
         PassParams prm = base_params(x, g);
-        prm.ingress = 1;
+        prm.ingress = x.ingress;
         prm.in[0] = d_a;
         prm.in[1] = d_b;
         prm.in_len[0] = la;
@@ -185,7 +186,7 @@ int run_transforms(const Xform& x, const void* d_a, const void* d_b, int64_t la,
                     prm.out[q] = buf[which];
                 }
                 if (j == 0) prm.bdesc = x.bdesc;
-                prm.ingress = j == 0;
+                prm.ingress = j == 0 ? x.ingress : 0;
                 prm.nops = nops;
                 PassLaunch L = {BW_MODE_FWD, g.logT, g.G, x.limb_bits, nper, nops, g.tiles};
                 LAUNCH_TRY(bw_launch_pass(L, prm, st));

@@ -20,6 +20,7 @@ struct DevTables {
     SetTables t;
     GarnerDev garner;
     F64Prime fp[3];
+    GarnerD64Dev gd;      // five-prime sets only
 };
 // Tables of prime set `set_id` on the current device (built on first use).
 int get_tables(int set_id, DevTables** out);
@@ -40,6 +41,7 @@ struct Xform {
     void* bufB = nullptr;
     const void* ninv = nullptr;     // per-prime N^-1 pairs for this N
     int schedule = 0;               // BW_SCHED_*
+    int ingress = 1;                // raw operands: 1 a limb per digit, 2 64-bit digits of 32-bit limb pairs
     const BwIngress* bdesc = nullptr;   // stacked ingress (B > 1), device memory
 };
 

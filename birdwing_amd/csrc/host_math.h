@@ -8,18 +8,26 @@
 
 namespace bw {
 
-enum { SET_U32_REF = 0, SET_U32_WIDE = 1, SET_U64_REF = 2, SET_HYBRID = 3, SET_COUNT = 4 };
+enum { SET_U32_REF = 0, SET_U32_WIDE = 1, SET_U64_REF = 2, SET_HYBRID = 3, SET_D64 = 4, SET_COUNT = 5 };
 
 struct PrimeSet {
     int id;
     int limb_bits;       // arithmetic: 32 -> FP64 kernels, 64 -> integer kernels
     int k;
-    uint64_t p[3];
-    uint64_t root_max[3];
+    uint64_t p[BW_MAX_PRIMES];
+    uint64_t root_max[BW_MAX_PRIMES];
+    int digit_bits;      // 0: a digit is one input limb; 64: 64-bit digits (pairs of 32-bit limbs)
 };
 
 const PrimeSet* prime_set(int id);
 int default_set(int limb_bits, uint64_t N);
+// Prime set a product without a requested set runs under: the 64-bit-digit
+// set when it supports the sizes, else default_set().
+int auto_set(int limb_bits, uint64_t L_A, uint64_t L_B);
+// Digits of an operand of L limbs of in_bits under set s.
+uint64_t digits_of(const PrimeSet& s, int in_bits, uint64_t L);
+// Transform length of a product under set s (from its digit counts).
+uint64_t padded_size_for(const PrimeSet& s, int in_bits, uint64_t L_A, uint64_t L_B);
 
 typedef unsigned __int128 u128;
 uint64_t mulmod(uint64_t a, uint64_t b, uint64_t p);
@@ -49,6 +57,7 @@ struct HostTables {
 };
 void build_tables(const PrimeSet& s, HostTables& out);
 GarnerDev garner_constants(const PrimeSet& s);
+GarnerD64Dev garner_d64_constants(const PrimeSet& s);
 void f64_prime(uint64_t p, F64Prime& out);
 // Per-plan (N^-1, aux) pairs, layout [k][2].
 void ninv_pairs(const PrimeSet& s, uint64_t N, std::vector<uint64_t>& out64, std::vector<double>& outf);

@@ -33,6 +33,10 @@ struct ArF64x {
     // words hold canonical [0, p) or centered residues; both read as int32
     __device__ static V from_word(W x, const PR&) { return (double)(int32_t)x; }
     __device__ static V from_raw(Raw x, const PR&) { return (double)x; }
+    // a 64-bit digit hi * 2^32 + lo: hi * (2^32 mod p) in (-p, p) plus lo (< 2^33 in all)
+    __device__ static V from_raw64(uint64_t d, const PC32& c) {
+        return f64_mulmod((double)(uint32_t)(d >> 32), c.c32[0], c.c32[1], c.P) + (double)(uint32_t)d;
+    }
     __device__ static W to_word(V x, const PR& P) { return f64_canon(x, P); }
     __device__ static W to_word_c(V x, const PR& P, bool canon) {
         return canon ? f64_canon(x, P) : (uint32_t)(int32_t)f64_reduce(x, P);
@@ -217,9 +221,10 @@ __device__ __forceinline__ typename AR::TW tw_lookup(int64_t e, const typename A
 
 // One workgroup = one tile of G lines (G a power of two, runtime) with
 // NT = G * R1 threads: thread i owns stage-0 group i and stage-1 groups
-// i, i + NT, ...  COLS (S > 1) and INGRESS (raw limbs in) are compile-time so
+// i, i + NT, ...  COLS (S > 1) and INGRESS (0: residue words in, 1: one raw
+// limb per digit, 2: 64-bit digits from pairs of 32-bit limbs) are compile-time so
 // the inner loops carry no branches.
-template <class AR, int LOGT, int MODE, bool COLS, bool INGRESS>
+template <class AR, int LOGT, int MODE, bool COLS, int INGRESS>
 __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     using V = typename AR::V;
     using W = typename AR::W;
@@ -263,8 +268,10 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
 
     // raw limbs of operand `which` for this thread's line; `base` becomes the
     // position inside the product (stacked products: line o reads product o)
-    auto ingress_src = [&](int which, const typename AR::Raw*& raw, int64_t& len, int64_t& base) {
-        raw = reinterpret_cast<const typename AR::Raw*>(prm.in[which]);
+    // raw limb type read by ingress: a 64-bit digit is two 32-bit limbs
+    using RL = typename std::conditional<INGRESS == 2, uint32_t, typename AR::Raw>::type;
+    auto ingress_src = [&](int which, const RL*& raw, int64_t& len, int64_t& base) {
+        raw = reinterpret_cast<const RL*>(prm.in[which]);
         len = prm.in_len[which];
         if (prm.bdesc) {
             const int64_t o = COLS ? o0 : o0 + g0;
@@ -274,20 +281,32 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
             base -= o << prm.logM;
         }
     };
-    // stage-0 input words of one operand (raw limbs or residue words)
-    using IW = typename std::conditional<INGRESS, typename AR::Raw, W>::type;
+    // stage-0 input words of one operand (raw digits or residue words)
+    using IW = typename std::conditional<INGRESS == 2, uint64_t,
+                                         typename std::conditional<INGRESS == 1, typename AR::Raw, W>::type>::type;
+    // digit a of a raw operand of len limbs (zero past the end)
+    auto raw_digit = [&](const RL* raw, int64_t len, int64_t a) -> IW {
+        if constexpr (INGRESS == 2) {
+            const uint64_t lo = 2 * a < len ? raw[2 * a] : 0u;
+            const uint64_t hi = 2 * a + 1 < len ? raw[2 * a + 1] : 0u;
+            return lo | (hi << 32);
+        } else {
+            return a < len ? (IW)raw[a] : (IW)0;
+        }
+    };
+    auto digit_value = [&](IW w) -> V {
+        if constexpr (INGRESS == 2) return AR::from_raw64(w, pc);
+        else return AR::from_raw(w, P);
+    };
     auto load0 = [&](int which, IW* w) {
         int64_t base = line_base(g0) + j0 * tstride;
         const int64_t step = (int64_t)R1 * tstride;
-        if constexpr (INGRESS) {
-            const typename AR::Raw* raw;
+        if constexpr (INGRESS != 0) {
+            const RL* raw;
             int64_t len;
             ingress_src(which, raw, len, base);
 #pragma unroll
-            for (int m = 0; m < R0; m++) {
-                const int64_t a = base + m * step;
-                w[m] = a < len ? raw[a] : (IW)0;
-            }
+            for (int m = 0; m < R0; m++) w[m] = raw_digit(raw, len, base + m * step);
         } else {
             const W* in = reinterpret_cast<const W*>(prm.in[which]) + plane_off;
 #pragma unroll
@@ -298,7 +317,7 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         V x[R0];
 #pragma unroll
         for (int m = 0; m < R0; m++) {
-            if constexpr (INGRESS) x[m] = AR::from_raw(w[m], P);
+            if constexpr (INGRESS != 0) x[m] = digit_value(w[m]);
             else x[m] = AR::from_word(w[m], P);
         }
         dft32_dif<AR, R0LOG>(x, pc);
@@ -313,15 +332,12 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         int64_t base = line_base(g0) + j0 * tstride;
         const int64_t step = (int64_t)R1 * tstride;
         V x[R0];
-        if constexpr (INGRESS) {
-            const typename AR::Raw* raw;
+        if constexpr (INGRESS != 0) {
+            const RL* raw;
             int64_t len;
             ingress_src(which, raw, len, base);
 #pragma unroll
-            for (int m = 0; m < R0; m++) {
-                const int64_t a = base + m * step;
-                x[m] = a < len ? AR::from_raw(raw[a], P) : AR::zero();
-            }
+            for (int m = 0; m < R0; m++) x[m] = digit_value(raw_digit(raw, len, base + m * step));
         } else {
             const W* in = reinterpret_cast<const W*>(prm.in[which]) + plane_off;
 #pragma unroll
@@ -489,20 +505,30 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
 typedef void (*PassKernel2)(PassParams);
 
 template <class AR, int MODE, int LOGT>
-static PassKernel2 pick2_v(bool cols, bool ingress) {
+static PassKernel2 pick2_v(bool cols, int ingress) {
+    constexpr bool D64 = std::is_same<AR, ArF64x>::value;   // 64-bit digits: FP64 arithmetic only
     if constexpr (MODE == BW_MODE_INV) {
-        return cols ? ntt2_kernel<AR, LOGT, MODE, true, false> : ntt2_kernel<AR, LOGT, MODE, false, false>;
+        if (ingress) return nullptr;
+        return cols ? ntt2_kernel<AR, LOGT, MODE, true, 0> : ntt2_kernel<AR, LOGT, MODE, false, 0>;
     } else if constexpr (MODE == BW_MODE_FUSED) {
         if (cols) return nullptr;
-        return ingress ? ntt2_kernel<AR, LOGT, MODE, false, true> : ntt2_kernel<AR, LOGT, MODE, false, false>;
+        if (ingress == 2) {
+            if constexpr (D64) return ntt2_kernel<AR, LOGT, MODE, false, 2>;
+            else return nullptr;
+        }
+        return ingress ? ntt2_kernel<AR, LOGT, MODE, false, 1> : ntt2_kernel<AR, LOGT, MODE, false, 0>;
     } else {
-        if (cols) return ingress ? ntt2_kernel<AR, LOGT, MODE, true, true> : ntt2_kernel<AR, LOGT, MODE, true, false>;
-        return ingress ? ntt2_kernel<AR, LOGT, MODE, false, true> : ntt2_kernel<AR, LOGT, MODE, false, false>;
+        if (ingress == 2) {
+            if constexpr (D64) return cols ? ntt2_kernel<AR, LOGT, MODE, true, 2> : ntt2_kernel<AR, LOGT, MODE, false, 2>;
+            else return nullptr;
+        }
+        if (cols) return ingress ? ntt2_kernel<AR, LOGT, MODE, true, 1> : ntt2_kernel<AR, LOGT, MODE, true, 0>;
+        return ingress ? ntt2_kernel<AR, LOGT, MODE, false, 1> : ntt2_kernel<AR, LOGT, MODE, false, 0>;
     }
 }
 
 template <class AR, int MODE>
-static PassKernel2 pick2(int logT, bool cols, bool ingress) {
+static PassKernel2 pick2(int logT, bool cols, int ingress) {
     switch (logT) {
     case 1: return pick2_v<AR, MODE, 1>(cols, ingress);
     case 2: return pick2_v<AR, MODE, 2>(cols, ingress);

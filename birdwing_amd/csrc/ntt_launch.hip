@@ -11,12 +11,12 @@ PassKernel bw_pick_f64_fused(int logT, int G);
 PassKernel bw_pick_u64_fwd(int logT, int G);
 PassKernel bw_pick_u64_inv(int logT, int G);
 PassKernel bw_pick_u64_fused(int logT, int G);
-PassKernel bw_pick2_f64_fwd(int logT, bool cols, bool ingress);
-PassKernel bw_pick2_f64_inv(int logT, bool cols, bool ingress);
-PassKernel bw_pick2_f64_fused(int logT, bool cols, bool ingress);
-PassKernel bw_pick2_u64_fwd(int logT, bool cols, bool ingress);
-PassKernel bw_pick2_u64_inv(int logT, bool cols, bool ingress);
-PassKernel bw_pick2_u64_fused(int logT, bool cols, bool ingress);
+PassKernel bw_pick2_f64_fwd(int logT, bool cols, int ingress);
+PassKernel bw_pick2_f64_inv(int logT, bool cols, int ingress);
+PassKernel bw_pick2_f64_fused(int logT, bool cols, int ingress);
+PassKernel bw_pick2_u64_fwd(int logT, bool cols, int ingress);
+PassKernel bw_pick2_u64_inv(int logT, bool cols, int ingress);
+PassKernel bw_pick2_u64_fused(int logT, bool cols, int ingress);
 
 int bw_pass_max_G(int logT, int limb_bits, bool cols) {
     if (logT <= BW_REG2_MAX_LOG) {
@@ -40,13 +40,13 @@ hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_
     const bool reg2 = L.logT <= BW_REG2_MAX_LOG;   // register-staged kernels
     if (reg2) {
         if (L.limb_bits == 64) {
-            if (L.mode == BW_MODE_FWD) k = bw_pick2_u64_fwd(L.logT, prm.S > 1, prm.ingress != 0);
-            else if (L.mode == BW_MODE_INV) k = bw_pick2_u64_inv(L.logT, prm.S > 1, prm.ingress != 0);
-            else k = bw_pick2_u64_fused(L.logT, prm.S > 1, prm.ingress != 0);
+            if (L.mode == BW_MODE_FWD) k = bw_pick2_u64_fwd(L.logT, prm.S > 1, prm.ingress);
+            else if (L.mode == BW_MODE_INV) k = bw_pick2_u64_inv(L.logT, prm.S > 1, prm.ingress);
+            else k = bw_pick2_u64_fused(L.logT, prm.S > 1, prm.ingress);
         } else {
-            if (L.mode == BW_MODE_FWD) k = bw_pick2_f64_fwd(L.logT, prm.S > 1, prm.ingress != 0);
-            else if (L.mode == BW_MODE_INV) k = bw_pick2_f64_inv(L.logT, prm.S > 1, prm.ingress != 0);
-            else k = bw_pick2_f64_fused(L.logT, prm.S > 1, prm.ingress != 0);
+            if (L.mode == BW_MODE_FWD) k = bw_pick2_f64_fwd(L.logT, prm.S > 1, prm.ingress);
+            else if (L.mode == BW_MODE_INV) k = bw_pick2_f64_inv(L.logT, prm.S > 1, prm.ingress);
+            else k = bw_pick2_f64_fused(L.logT, prm.S > 1, prm.ingress);
         }
     } else if (L.limb_bits == 64) {
         if (L.mode == BW_MODE_FWD) k = bw_pick_u64_fwd(L.logT, L.G);

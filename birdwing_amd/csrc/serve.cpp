@@ -343,7 +343,7 @@ extern "C" int bw_server_create(bw_server** out, const bw_server_config* cfg) {
         if (e != hipSuccess) rc = fail(BW_ERR_HIP, "server streams: %s", hipGetErrorString(e));
     }
     // tables of every prime set a request can need, before the first request
-    for (int set : {SET_U32_REF, SET_U32_WIDE, SET_U64_REF}) {
+    for (int set : {SET_U32_REF, SET_U32_WIDE, SET_U64_REF, SET_D64}) {
         DevTables* t;
         if (!rc) rc = get_tables(set, &t);
     }
@@ -364,7 +364,7 @@ extern "C" int64_t bw_server_submit(bw_server* s, int limb_bits, const void* h_a
     if (limb_bits != 32 && limb_bits != 64) return fail(BW_ERR_ARG, "limb_bits must be 32 or 64");
     if (L_A == 0 || L_B == 0) return fail(BW_ERR_UNSUPPORTED_SIZE, "operand limb count must be positive");
     {
-        const PrimeSet* ps = prime_set(default_set(limb_bits, padded_ntt_size(L_A, L_B)));
+        const PrimeSet* ps = prime_set(auto_set(limb_bits, L_A, L_B));
         std::string why;
         if (!ps || !multiply_size_supported(*ps, limb_bits, L_A, L_B, &why))
             return fail(BW_ERR_UNSUPPORTED_SIZE, "unsupported multiply: %s", why.c_str());

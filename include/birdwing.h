@@ -13,13 +13,23 @@
  * reference, include/config.h:17-53):
  *   limb_bits = 32: 32-bit limbs, three ~30-bit primes, 128-bit Garner.
  *   limb_bits = 64: 64-bit limbs, three 58-bit primes, 160-bit Garner.
- * Prime sets (src/gpu_ntt.cu:28-47 plus one wide 32-bit set):
+ * Prime sets (src/gpu_ntt.cu:28-47 plus two of this library's):
  *   BW_SET_U32_REF  {0x2d000001, 0x23800001, 0x26800001}, N <= 2^23
  *   BW_SET_U32_WIDE {2013265921, 1811939329, 469762049},  N <= 2^26
  *   BW_SET_U64_REF  {0x400002600000001, 0x400004200000001, 0x400001100000001}
  *   BW_SET_HYBRID   first two u64 primes (u32 limbs; reserved)
- * Passing prime_set = -1 picks BW_SET_U32_REF while N <= 2^23, BW_SET_U32_WIDE
- * above (u32), BW_SET_U64_REF (u64).
+ *   BW_SET_D64      {2113929217, 2013265921, 1811939329, 1711276033, 1107296257}:
+ *                   64-bit digits (a pair of 32-bit limbs or one 64-bit limb),
+ *                   five transforms of half the length, FP64 arithmetic,
+ *                   digit transforms of up to 2^25 points.
+ * bw_default_prime_set() is the three-prime set of the reference pipeline for
+ * a 32-bit-digit length N: BW_SET_U32_REF while N <= 2^23, BW_SET_U32_WIDE
+ * above (u32), BW_SET_U64_REF (u64).  A plan created with prime_set = -1 runs
+ * under BW_SET_D64 when that set supports the sizes (fewer butterflies and
+ * residue bytes for the same product) and under the default set otherwise;
+ * every product is exact either way, but only three-prime plans give the
+ * reference's CRT coefficients (bw_execute_crt).  Arrays receiving primes
+ * hold at least 5 entries.
  *
  * Streams are hipStream_t values passed as void* (NULL = default stream).
  * Device entry points are asynchronous on that stream; *_host entry points
@@ -48,7 +58,8 @@ enum {
 /* Launch order of a plan's passes (bw_plan_set_schedule). */
 enum { BW_SCHED_PASS_MAJOR = 0, BW_SCHED_PRIME_MAJOR = 1 };
 
-enum { BW_SET_DEFAULT = -1, BW_SET_U32_REF = 0, BW_SET_U32_WIDE = 1, BW_SET_U64_REF = 2, BW_SET_HYBRID = 3 };
+enum { BW_SET_DEFAULT = -1, BW_SET_U32_REF = 0, BW_SET_U32_WIDE = 1, BW_SET_U64_REF = 2, BW_SET_HYBRID = 3,
+       BW_SET_D64 = 4 };
 
 typedef struct bw_plan bw_plan;
 
@@ -56,7 +67,7 @@ typedef struct bw_plan_info {
     int limb_bits;
     int prime_set;
     int k;                 /* number of primes */
-    uint64_t primes[3];
+    uint64_t primes[5];
     uint64_t L_A, L_B;
     uint64_t N;            /* next_pow2(L_A + L_B - 1), the reference's padded size */
     uint64_t N_exec;       /* transform length actually run (>= 2) */
@@ -64,9 +75,14 @@ typedef struct bw_plan_info {
     int pass_log[4];       /* log2 of each pass's points */
     int nlaunch;           /* kernel launches per bw_execute */
     uint64_t device_bytes; /* device memory held by the plan */
+    int digit_bits;        /* 64 for the five-prime 64-bit-digit set, else limb_bits */
 } bw_plan_info;
 
 /* ---- library ---------------------------------------------------------- */
+/* Layout version of the structs in this header; a caller built against one
+ * version must not run against a library reporting another. */
+#define BW_ABI_VERSION 2
+int bw_abi_version(void);
 const char* bw_last_error(void);
 const char* bw_version(void);
 /* hipSetDevice + table warm-up for the device; optional. */

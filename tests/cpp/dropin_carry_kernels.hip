@@ -20,6 +20,11 @@
 
 int main() {
     int fails = 0;
+    if (bw_abi_version() != BW_ABI_VERSION) {
+        std::fprintf(stderr, "ABI mismatch: header %d, library %d (rebuild tests/cpp)\n", BW_ABI_VERSION,
+                     bw_abi_version());
+        return 3;
+    }
     std::mt19937_64 rng(5);
     for (size_t N : {size_t(1), size_t(1000), size_t(1024), size_t(5000), size_t(70001)}) {
         for (int mode = 0; mode < 2; mode++) {  // 0: random, 1: all-ones words (long carry chains)

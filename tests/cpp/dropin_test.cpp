@@ -254,6 +254,11 @@ static void gpu_checks() {
 
 int main(int argc, char** argv) {
     const bool gpu = argc > 1 && std::strcmp(argv[1], "--gpu") == 0;
+    if (bw_abi_version() != BW_ABI_VERSION) {   // built against another header than the library's
+        std::fprintf(stderr, "ABI mismatch: header %d, library %d (rebuild tests/cpp)\n", BW_ABI_VERSION,
+                     bw_abi_version());
+        return 3;
+    }
     host_checks();
     if (gpu) gpu_checks();
     std::printf("dropin_test LIMB_BITS=%d native=%d: %s (%d failures)\n", LIMB_BITS,

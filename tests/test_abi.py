@@ -75,3 +75,58 @@ def test_default_prime_sets():
     assert bw.default_prime_set(32, 1 << 25) == 1
     assert bw.default_prime_set(64, 1 << 25) == 2
     assert bw.prime_set_info(1)[0] == [2013265921, 1811939329, 469762049]
+
+
+def _is_prime(n):
+    if n < 2:
+        return False
+    for q in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        if n % q == 0:
+            return n == q
+    d, s = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        s += 1
+    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(s - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def test_d64_prime_set():
+    """The five-prime 64-bit-digit set: primes c*2^m+1 < 2^31 with c < 128 and
+    m >= 25, roots of exact order 2^m, product above the coefficient bound of
+    2^24-digit operands (2^24 * (2^64-1)^2)."""
+    import birdwing_amd as bw
+    primes, roots = bw.prime_set_info(bw.SET_D64)
+    assert len(primes) == 5
+    prod = 1
+    for p, r in zip(primes, roots):
+        assert _is_prime(p) and p < 2**31
+        m = ((p - 1) & -(p - 1)).bit_length() - 1
+        assert m >= 25 and (p - 1) >> m < 128
+        assert pow(r, 1 << m, p) == 1 and pow(r, 1 << (m - 1), p) == p - 1
+        prod *= p
+    assert prod > (1 << 24) * (2**64 - 1) ** 2
+    assert bw.max_supported_logN(bw.SET_D64) == 25
+    # 32-bit limbs pair into digits: 2^25 limbs per operand fit, one more does not
+    assert bw.multiply_size_supported(bw.SET_D64, 32, 1 << 25, 1 << 25)[0]
+    assert not bw.multiply_size_supported(bw.SET_D64, 32, (1 << 25) + 1, (1 << 25) + 1)[0]
+    assert bw.multiply_size_supported(bw.SET_D64, 64, 1 << 24, 1 << 24)[0]
+    assert not bw.multiply_size_supported(bw.SET_D64, 64, (1 << 24) + 1, (1 << 24) + 1)[0]
+    assert bw.max_supported_limb_count(bw.SET_D64, 32) == 1 << 25
+
+
+def test_abi_version_matches_header():
+    import re
+    import birdwing_amd as bw
+    hdr = open(os.path.join(ROOT, "include", "birdwing.h")).read()
+    v = int(re.search(r"#define BW_ABI_VERSION (\d+)", hdr).group(1))
+    assert bw.lib().bw_abi_version() == v

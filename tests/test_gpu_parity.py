@@ -319,10 +319,21 @@ def test_product_pass_splits(gpu, oracle, passes):
     """Every pass decomposition gives the same product (N = 2^13)."""
     A = oracle.uniform_limbs(4096, 11)
     B = oracle.uniform_limbs(4096, 12)
-    _product_check(gpu, oracle, A, B, 32, -1, passes)
+    _product_check(gpu, oracle, A, B, 32, gpu.default_prime_set(32, 8192), passes)
     A = oracle.uniform_limbs(4096, 13, 64)
     B = oracle.uniform_limbs(4096, 14, 64)
-    _product_check(gpu, oracle, A, B, 64, -1, passes)
+    _product_check(gpu, oracle, A, B, 64, gpu.default_prime_set(64, 8192), passes)
+
+
+@pytest.mark.parametrize("passes", [[6, 6], [4, 8], [8, 4], [2, 10], [10, 2], [3, 4, 5], [1, 1, 10]])
+def test_d64_pass_splits(gpu, oracle, passes):
+    """Every pass decomposition of the 64-bit-digit transform (N = 2^12 digits)."""
+    A = oracle.uniform_limbs(4096, 21)
+    B = oracle.uniform_limbs(4095, 22)
+    _product_check(gpu, oracle, A, B, 32, gpu.SET_D64, passes)
+    A = oracle.uniform_limbs(2048, 23, 64)
+    B = oracle.uniform_limbs(2047, 24, 64)
+    _product_check(gpu, oracle, A, B, 64, gpu.SET_D64, passes)
 
 
 @pytest.mark.parametrize("schedule", [0, 1])
@@ -347,7 +358,7 @@ def test_execute_crt_matches_oracle(gpu, oracle, bits, L):
     d = dev()
     A = oracle.ref_pipeline_limbs(L, 100 + L) if bits == 32 else oracle.uniform_limbs(L, 100 + L, 64)
     B = oracle.ref_pipeline_limbs(L, 101 + L) if bits == 32 else oracle.uniform_limbs(L, 101 + L, 64)
-    plan = gpu.Plan(bits, L, L)
+    plan = gpu.Plan(bits, L, L, gpu.default_prime_set(bits, gpu.padded_ntt_size(L, L)))
     N = plan.N_exec
     c0, c1 = d.empty(N, 64), d.empty(N, 64)
     c2 = d.empty(N, 32) if bits == 64 else None
@@ -392,3 +403,40 @@ def test_plan_autotune_keeps_product(gpu, oracle, bits, L):
     assert np.array_equal(d.from_dev(c, len(A) + len(B)), want), plan.passes
     # a new plan of the same shape starts from the tuned split
     assert gpu.Plan(bits, len(A), len(B)).passes == plan.passes
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("LA,LB", [(1, 1), (1, 2), (2, 3), (3, 5), (7, 7), (2047, 2049), (4096, 4096),
+                                   (10001, 9999), (1 << 16, (1 << 16) + 1), (100000, 3), (1 << 18, 1 << 18)])
+def test_d64_products_vs_gmp(gpu, oracle, bits, LA, LB):
+    """64-bit digits under five primes (the automatic choice) against GMP:
+    odd and even 32-bit limb counts, unbalanced operands."""
+    A = oracle.uniform_limbs(LA, LA * 7 + 1, bits)
+    B = oracle.uniform_limbs(LB, LB * 11 + 2, bits)
+    plan = _product_check(gpu, oracle, A, B, bits)
+    assert plan.prime_set == gpu.SET_D64 and plan.digit_bits == 64
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_d64_carry_stress(gpu, oracle, bits):
+    """All-ones operands (longest carry chains) and (2^b - 1)^2 under 64-bit digits."""
+    for L in (4097, 30001):
+        A = np.full(L, np.iinfo(np.uint64 if bits == 64 else np.uint32).max,
+                    dtype=np.uint64 if bits == 64 else np.uint32)
+        plan = _product_check(gpu, oracle, A, A, bits)
+        assert plan.prime_set == gpu.SET_D64
+    b = bits * 5001 - 3
+    x = (1 << b) - 1
+    n = (b + bits - 1) // bits
+    A = oracle.from_int(x, n, bits)
+    _product_check(gpu, oracle, A, A, bits)
+
+
+def test_d64_execute_crt_refused(gpu, oracle):
+    plan = gpu.Plan(32, 1000, 1000)
+    assert plan.prime_set == gpu.SET_D64
+    d = dev()
+    c0, c1 = d.empty(plan.N_exec, 64), d.empty(plan.N_exec, 64)
+    A = oracle.uniform_limbs(1000, 5, 32)
+    with pytest.raises(gpu.BWError):
+        plan.execute_crt(d.to_dev(A), d.to_dev(A), c0, c1)
