@@ -440,3 +440,72 @@ def test_d64_execute_crt_refused(gpu, oracle):
     A = oracle.uniform_limbs(1000, 5, 32)
     with pytest.raises(gpu.BWError):
         plan.execute_crt(d.to_dev(A), d.to_dev(A), c0, c1)
+
+
+# ------------------------------------------------ the measured configurations
+@pytest.mark.parametrize("pset", [-1, "ref3"])
+def test_config_plain_2048(gpu, oracle, pset):
+    """2048 x 2048 32-bit limbs, seeded uniform, top limb nonzero: under the
+    reference's 32-bit prime set (N = 4096) and under the automatic choice."""
+    A = oracle.uniform_limbs(2048, 1, 32)
+    B = oracle.uniform_limbs(2048, 2, 32)
+    s = gpu.SET_U32_REF if pset == "ref3" else -1
+    plan = _product_check(gpu, oracle, A, B, 32, s)
+    if pset == "ref3":
+        assert plan.N == 4096 and plan.primes == [0x2d000001, 0x23800001, 0x26800001]
+
+
+@pytest.mark.parametrize("pset", [-1, "ref3"])
+def test_config_carry_edge_stress(gpu, oracle, pset):
+    """Carry/edge stress at full size (32-bit limbs): unbalanced 2^22 x 2^14,
+    all-ones 2^22-limb operands, 99%-zero operands, (2^b - 1)^2 for
+    b = 32 * 2^22 - 7 and b = 32 * 2^22; under the reference pipeline's
+    three-prime set and under the automatic (five-prime) choice."""
+    L = 1 << 22
+
+    def set_for(la, lb):
+        return gpu.default_prime_set(32, gpu.padded_ntt_size(la, lb)) if pset == "ref3" else -1
+
+    A = oracle.uniform_limbs(L, 31, 32)
+    B = oracle.uniform_limbs(1 << 14, 32, 32)
+    print("2^22 x 2^14", flush=True)
+    _product_check(gpu, oracle, A, B, 32, set_for(L, 1 << 14))
+    O = np.full(L, 0xFFFFFFFF, np.uint32)
+    print("all-ones 2^22", flush=True)
+    _product_check(gpu, oracle, O, O, 32, set_for(L, L))
+    rng = np.random.default_rng(77)
+    Z1 = np.where(rng.random(L) < 0.01, rng.integers(1, 2**32, L, dtype=np.uint64), 0).astype(np.uint32)
+    Z2 = np.where(rng.random(L) < 0.01, rng.integers(1, 2**32, L, dtype=np.uint64), 0).astype(np.uint32)
+    Z1[-1] |= 1
+    Z2[-1] |= 1
+    print("99% zero 2^22", flush=True)
+    _product_check(gpu, oracle, Z1, Z2, 32, set_for(L, L))
+    for b in (32 * L - 7, 32 * L):
+        n = (b + 31) // 32
+        X = np.full(n, 0xFFFFFFFF, np.uint32)
+        if b % 32:
+            X[-1] = (1 << (b % 32)) - 1
+        print("(2^b-1)^2 b=%d" % b, flush=True)
+        _product_check(gpu, oracle, X, X, 32, set_for(n, n))
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_config_plain_large(gpu, oracle, bits):
+    """2^24 x 2^24 limbs (N = 2^25 32-bit digits), seeded uniform, vs GMP."""
+    L = 1 << 24
+    A = oracle.uniform_limbs(L, 41, bits)
+    B = oracle.uniform_limbs(L, 42, bits)
+    print("2^24 x 2^24 u%d" % bits, flush=True)
+    _product_check(gpu, oracle, A, B, bits)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("LA,LB", [(1, 1), (3, 5), (1000, 1003), (4097, 4095), (1 << 15, 1 << 15),
+                                   (70000, 1000), (1 << 18, (1 << 18) - 1)])
+def test_three_prime_products_vs_gmp(gpu, oracle, bits, LA, LB):
+    """The reference pipelines' three-prime sets (chosen explicitly) against GMP."""
+    A = oracle.uniform_limbs(LA, LA * 3 + 7, bits)
+    B = oracle.uniform_limbs(LB, LB * 5 + 9, bits)
+    s = gpu.default_prime_set(bits, gpu.padded_ntt_size(LA, LB))
+    plan = _product_check(gpu, oracle, A, B, bits, s)
+    assert plan.k == 3
