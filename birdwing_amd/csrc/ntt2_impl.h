@@ -64,6 +64,20 @@ struct ArF64x {
         return f64_mulmod(r, ninv[0], ninv[1], P);
     }
     __device__ static V zero() { return 0.0; }
+    // Fused pass: A's spectrum waits in registers as centered int32 residues
+    // (half the registers of FP64) and then serves as the reduced multiplier
+    // of the pointwise product, so B's unreduced value needs no reduction.
+    using KV = int32_t;
+    __device__ static KV to_keep(V x, const PR& P) { return (int32_t)f64_reduce(x, P); }
+    __device__ static V from_keep(KV k) { return (double)k; }
+    __device__ static KV keep_mul(KV k, V x, const PR& P) {
+        const double w = (double)k;
+        return (int32_t)f64_mulmod(x, w, w * P.pinv, P);
+    }
+    __device__ static KV keep_mul_scaled(KV k, V x, const NI* ninv, const PR& P) {
+        const double w = (double)k;
+        return (int32_t)f64_mulmod(f64_mulmod(x, w, w * P.pinv, P), ninv[0], ninv[1], P);
+    }
     struct Gen {          // w_k = base * step^k, values in (-p, p)
         double w, step, stepp;
         __device__ void init(double2 base, double2 st, const PR&) { w = base.x; step = st.x; stepp = st.y; }
@@ -127,6 +141,11 @@ struct ArU64x {
         return u64_montmul(u64_montmul(a, b, P), ninv[0], P);
     }
     __device__ static V zero() { return 0; }
+    using KV = uint64_t;
+    __device__ static KV to_keep(V x, const PR&) { return x; }
+    __device__ static V from_keep(KV k) { return k; }
+    __device__ static KV keep_mul(KV k, V x, const PR& P) { return mul_plain(k, x, P); }
+    __device__ static KV keep_mul_scaled(KV k, V x, const NI* ninv, const PR& P) { return mul_scaled(k, x, ninv, P); }
     struct Gen {          // Montgomery form: wm = w 2^64 mod p
         uint64_t wm, stepm;
         __device__ void init(ulonglong2 base, ulonglong2 st, const PR& P) {
@@ -284,20 +303,15 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     // stage-0 input words of one operand (raw digits or residue words)
     using IW = typename std::conditional<INGRESS == 2, uint64_t,
                                          typename std::conditional<INGRESS == 1, typename AR::Raw, W>::type>::type;
-    // digit a of a raw operand of len limbs (zero past the end)
-    auto raw_digit = [&](const RL* raw, int64_t len, int64_t a) -> IW {
-        if constexpr (INGRESS == 2) {
-            const uint64_t lo = 2 * a < len ? raw[2 * a] : 0u;
-            const uint64_t hi = 2 * a + 1 < len ? raw[2 * a + 1] : 0u;
-            return lo | (hi << 32);
-        } else {
-            return a < len ? (IW)raw[a] : (IW)0;
-        }
-    };
     auto digit_value = [&](IW w) -> V {
         if constexpr (INGRESS == 2) return AR::from_raw64(w, pc);
         else return AR::from_raw(w, P);
     };
+    // This thread's R0 stage-0 input words of one operand: digits base + m*step.
+    // Raw operands are zero past their end.  Three cases: every digit inside
+    // (loads without per-element tests, 64-bit when the limb pair is 8-byte
+    // aligned), every digit past the end (no loads), and digits straddling the
+    // end (per-element tests).
     auto load0 = [&](int which, IW* w) {
         int64_t base = line_base(g0) + j0 * tstride;
         const int64_t step = (int64_t)R1 * tstride;
@@ -305,8 +319,43 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
             const RL* raw;
             int64_t len;
             ingress_src(which, raw, len, base);
+            constexpr int LW = INGRESS == 2 ? 2 : 1;          // limbs per digit
+            const int64_t last = base + (int64_t)(R0 - 1) * step;
+            if (LW * last + (LW - 1) < len) {
+                if constexpr (INGRESS == 2) {
+                    if ((reinterpret_cast<uintptr_t>(raw) & 7) == 0) {
+                        const uint64_t* r64 = reinterpret_cast<const uint64_t*>(raw);
 #pragma unroll
-            for (int m = 0; m < R0; m++) w[m] = raw_digit(raw, len, base + m * step);
+                        for (int m = 0; m < R0; m++) w[m] = r64[base + m * step];
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < R0; m++) {
+                            const int64_t a = 2 * (base + m * step);
+                            w[m] = (uint64_t)raw[a] | ((uint64_t)raw[a + 1] << 32);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < R0; m++) w[m] = (IW)raw[base + m * step];
+                }
+            } else if (LW * base >= len) {
+#pragma unroll
+                for (int m = 0; m < R0; m++) w[m] = (IW)0;
+            } else {
+                // straddles the end: digits below dfull are whole, digit dfull
+                // may hold only its low limb (odd limb count)
+                const int64_t dfull = len / LW;
+#pragma unroll
+                for (int m = 0; m < R0; m++) {
+                    const int64_t a = base + m * step;
+                    if constexpr (INGRESS == 2) {
+                        w[m] = a < dfull ? ((uint64_t)raw[2 * a] | ((uint64_t)raw[2 * a + 1] << 32))
+                                         : (a == dfull && (len & 1)) ? (uint64_t)raw[2 * a] : (uint64_t)0;
+                    } else {
+                        w[m] = a < dfull ? (IW)raw[a] : (IW)0;
+                    }
+                }
+            }
         } else {
             const W* in = reinterpret_cast<const W*>(prm.in[which]) + plane_off;
 #pragma unroll
@@ -329,33 +378,15 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
                 k == 0 ? AR::to_lds(x[0], P) : AR::to_lds_r(x[brev(k, R0LOG)]);
     };
     auto fwd_stage0 = [&](int which) {
-        int64_t base = line_base(g0) + j0 * tstride;
-        const int64_t step = (int64_t)R1 * tstride;
-        V x[R0];
-        if constexpr (INGRESS != 0) {
-            const RL* raw;
-            int64_t len;
-            ingress_src(which, raw, len, base);
-#pragma unroll
-            for (int m = 0; m < R0; m++) x[m] = digit_value(raw_digit(raw, len, base + m * step));
-        } else {
-            const W* in = reinterpret_cast<const W*>(prm.in[which]) + plane_off;
-#pragma unroll
-            for (int m = 0; m < R0; m++) x[m] = AR::from_word(in[base + m * step], P);
-        }
-        dft32_dif<AR, R0LOG>(x, pc);
-#pragma unroll
-        for (int k = 1; k < R0; k++) x[brev(k, R0LOG)] = AR::mul(x[brev(k, R0LOG)], tws[(j0 * k) << TWSH], P);
-#pragma unroll
-        for (int k = 0; k < R0; k++)   // k > 0 left a twiddle product in (-p, p)
-            lds[g0 * LT + T2Lds<LOGT, COLS>::pos(k * R1 + j0)] =
-                k == 0 ? AR::to_lds(x[0], P) : AR::to_lds_r(x[brev(k, R0LOG)]);
+        IW w[R0];
+        load0(which, w);
+        fwd_stage0_from(w);
     };
     // fused pass: A's stage-1 spectrum, then the pointwise product, held in registers
     constexpr int KEEPN = IT1 * R1;
-    V keep[KEEPN];
+    typename AR::KV keep[KEEPN];
 #pragma unroll
-    for (int i = 0; i < KEEPN; i++) keep[i] = AR::zero();
+    for (int i = 0; i < KEEPN; i++) keep[i] = 0;
     auto dec1 = [&](int gi, int& g, int& b) {
         if (COLS) { g = gi & (G - 1); b = gi >> logG; } else { g = gi >> R0LOG; b = gi & (R0 - 1); }
     };
@@ -376,16 +407,16 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
             dft32_dif<AR, R1LOG>(x, pc);
             if constexpr (KM == 1) {
 #pragma unroll
-                for (int k = 0; k < R1; k++) keep[it * R1 + k] = x[brev(k, R1LOG)];
+                for (int k = 0; k < R1; k++) keep[it * R1 + k] = AR::to_keep(x[brev(k, R1LOG)], P);
                 continue;
             } else if constexpr (KM == 2) {
 #pragma unroll
-                for (int k = 0; k < R1; k++) keep[it * R1 + k] = AR::mul_plain(keep[it * R1 + k], x[brev(k, R1LOG)], P);
+                for (int k = 0; k < R1; k++) keep[it * R1 + k] = AR::keep_mul(keep[it * R1 + k], x[brev(k, R1LOG)], P);
                 continue;
             } else if constexpr (KM == 3) {
 #pragma unroll
                 for (int k = 0; k < R1; k++)
-                    keep[it * R1 + k] = AR::mul_scaled(keep[it * R1 + k], x[brev(k, R1LOG)], ninv, P);
+                    keep[it * R1 + k] = AR::keep_mul_scaled(keep[it * R1 + k], x[brev(k, R1LOG)], ninv, P);
                 continue;
             }
             W* out = reinterpret_cast<W*>(prm.out[which]) + plane_off + line_base(g) + (int64_t)b * R1 * tstride;
@@ -422,7 +453,7 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
             V x[R1];
             if constexpr (KEEP) {
 #pragma unroll
-                for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = keep[it * R1 + k];
+                for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::from_keep(keep[it * R1 + k]);
             } else {
                 const W* in = reinterpret_cast<const W*>(prm.in[0]) + plane_off + line_base(g) + (int64_t)b * R1 * tstride;
                 if constexpr (COLS) {
