@@ -254,13 +254,14 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     constexpr int LT = t2_lds_line(LOGT, COLS);   // the host sizes LDS with the same function
     constexpr int IT1 = R0 > R1 ? R0 / R1 : 1;     // stage-1 groups per thread (upper bound)
     constexpr int TWSH = BW_STAGE_LOG - LOGT;      // w_T = w_8192^(2^TWSH)
-    constexpr int TWMASK = (1 << BW_STAGE_LOG) - 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     typename AR::LV* lds = reinterpret_cast<typename AR::LV*>(smem_raw);
 
     const int NT = blockDim.x;
     const int logG = __builtin_ctz(NT) - R1LOG;
     const int G = 1 << logG;
+    // stage twiddles w_T^e (e < T) staged once per workgroup after the tile
+    TW* twl = reinterpret_cast<TW*>(smem_raw + t2_tile_bytes(G, LT, (int)sizeof(typename AR::LV)));
     const int op = (MODE == BW_MODE_FWD) ? blockIdx.z : 0;
     const int bx = blockIdx.x, prime = prm.prime0 + blockIdx.y;
     const typename AR::PC32 pc = reinterpret_cast<const typename AR::PC32*>(prm.consts)[prime];
@@ -371,7 +372,7 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         }
         dft32_dif<AR, R0LOG>(x, pc);
 #pragma unroll
-        for (int k = 1; k < R0; k++) x[brev(k, R0LOG)] = AR::mul(x[brev(k, R0LOG)], tws[(j0 * k) << TWSH], P);
+        for (int k = 1; k < R0; k++) x[brev(k, R0LOG)] = AR::mul(x[brev(k, R0LOG)], twl[j0 * k], P);
 #pragma unroll
         for (int k = 0; k < R0; k++)   // k > 0 left a twiddle product in (-p, p)
             lds[g0 * LT + T2Lds<LOGT, COLS>::pos(k * R1 + j0)] =
@@ -483,7 +484,7 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         for (int k = 0; k < R0; k++) x[brev(k, R0LOG)] = AR::from_lds(lds[g0 * LT + T2Lds<LOGT, COLS>::pos(k * R1 + j0)]);
 #pragma unroll
         for (int k = 1; k < R0; k++)
-            x[brev(k, R0LOG)] = AR::mul(x[brev(k, R0LOG)], tws[(-((j0 * k) << TWSH)) & TWMASK], P);
+            x[brev(k, R0LOG)] = AR::mul(x[brev(k, R0LOG)], twl[(T - j0 * k) & (T - 1)], P);
         dft32_dit_inv<AR, R0LOG>(x, pc);
         W* out = reinterpret_cast<W*>(prm.out[0]) + plane_off + line_base(g0) + j0 * tstride;
         const int64_t step = (int64_t)R1 * tstride;
@@ -496,11 +497,19 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         }
     };
 
+    auto fill_twiddles = [&]() {
+        for (int i = tid; i < T; i += NT) twl[i] = tws[i << TWSH];
+    };
     if constexpr (MODE == BW_MODE_FWD) {
-        fwd_stage0(op);
+        IW w[R0];
+        load0(op, w);
+        fill_twiddles();
+        __syncthreads();
+        fwd_stage0_from(w);
         __syncthreads();
         fwd_stage1(op, std::integral_constant<int, 0>());
     } else if constexpr (MODE == BW_MODE_INV) {
+        fill_twiddles();
         inv_stage1(std::integral_constant<int, 0>());
         __syncthreads();
         inv_stage0();
@@ -513,9 +522,15 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
             IW wa[R0];
             load0(0, wa);
             load0(1, wb);
+            fill_twiddles();
+            __syncthreads();
             fwd_stage0_from(wa);
         } else {
-            fwd_stage0(0);
+            IW wa[R0];
+            load0(0, wa);
+            fill_twiddles();
+            __syncthreads();
+            fwd_stage0_from(wa);
         }
         __syncthreads();
         fwd_stage1(0, std::integral_constant<int, 1>());

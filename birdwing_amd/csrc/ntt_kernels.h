@@ -79,6 +79,12 @@ __host__ __device__ constexpr int t2_lds_line(int logT, bool cols) {
                : (1 << logT) + ((1 << logT) >> t2_r1log(logT, cols)) + 1;
 }
 int bw_reg2_lds_line(int logT, bool cols);
+// LDS of a register-staged workgroup: the tile (G lines of LT values of vbytes,
+// rounded up to 16 bytes), then the T stage twiddles (16-byte pairs).
+__host__ __device__ constexpr int t2_tile_bytes(int G, int lt, int vbytes) { return (G * lt * vbytes + 15) / 16 * 16; }
+__host__ __device__ constexpr int t2_lds_bytes(int logT, bool cols, int G, int vbytes) {
+    return t2_tile_bytes(G, t2_lds_line(logT, cols), vbytes) + (16 << logT);
+}
 // Frequency held at position q of a pass line (either kernel family).
 __host__ __device__ inline int64_t bw_pass_freq_of_pos(int logT, int64_t q, bool cols) {
     if (logT <= BW_REG2_MAX_LOG) {
