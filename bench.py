@@ -52,17 +52,28 @@ def launch_names(plan, schedule):
     if P == 1:
         return [("fused_pass(T=2^%d)" % logs[0], k), ("garner_carry", k)]
     out = []
-    chunks = range(k) if schedule == 1 else [None]
-    for c in chunks:
-        tag = "" if c is None else "[p%d]" % c
-        kk = 1 if c is not None else k
-        ops = ["A", "B"] if c is not None else ["AB"]
+
+    def forward(tag, kk, ops):
         for o in ops:
             for j in range(P - 1):
                 out.append(("fwd_pass%d(T=2^%d)%s%s" % (j + 1, logs[j], tag, o), kk * (2 if o == "AB" else 1)))
+
+    def backward(tag, kk):
         out.append(("fused_pass%d(T=2^%d)%s" % (P, logs[P - 1], tag), kk))
         for j in range(P - 2, -1, -1):
             out.append(("inv_pass%d(T=2^%d)%s" % (j + 1, logs[j], tag), kk))
+
+    if schedule == 1:          # prime-major
+        for c in range(k):
+            forward("[p%d]" % c, 1, ["A", "B"])
+            backward("[p%d]" % c, 1)
+    elif schedule == 2:        # hybrid: forward pass-major, the rest per prime
+        forward("", k, ["AB"])
+        for c in range(k):
+            backward("[p%d]" % c, 1)
+    else:
+        forward("", k, ["AB"])
+        backward("", k)
     out.append(("garner_carry", k))
     return out
 
@@ -446,7 +457,8 @@ def main():
     ap.add_argument("--limbs", type=int, default=1 << 24, help="limbs per operand")
     ap.add_argument("--bits", type=int, default=32, choices=(32, 64))
     ap.add_argument("--passes", type=str, default="", help="pass split, e.g. 12,13")
-    ap.add_argument("--schedule", type=int, default=0, help="0 pass-major (default), 1 prime-major launches")
+    ap.add_argument("--schedule", type=int, default=-1,
+                    help="launch order: 0 pass-major, 1 prime-major, 2 hybrid (default: the autotuner's choice)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-tune", action="store_true", help="keep the default pass split (no autotuning)")
     ap.add_argument("--no-oracle", action="store_true", help="GMP only as CPU baseline")
@@ -485,12 +497,14 @@ def main():
     db = torch.from_numpy(B.view(ndt)).cuda()
     dc = torch.empty(2 * L, dtype=tdt, device="cuda")
     plan = bw.Plan(bits, L, L, bw.SET_DEFAULT, passes)
-    plan.set_schedule(args.schedule)
     stream = torch.cuda.current_stream()
     tuned = False
     if not passes and not args.no_tune:
-        plan.tune(da, db, stream, reps=2)      # pass split / tile cap, before any timing
+        plan.tune(da, db, stream, reps=2)      # pass split / tile cap / launch order, before any timing
         tuned = True
+    if args.schedule >= 0:
+        plan.set_schedule(args.schedule)
+    schedule = int(plan.info.schedule)
 
     for _ in range(args.warmup):
         plan.execute(da, db, dc, stream)
@@ -515,7 +529,7 @@ def main():
     lat = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps))
     p50 = float(np.percentile(lat, 50))
     p99 = float(np.percentile(lat, 99))
-    nl = launch_names(plan, args.schedule)
+    nl = launch_names(plan, schedule)
     names = [n for n, _ in nl]
     lbytes = launch_bytes(plan, bits, nl)
     # aggregate launches of one kernel type (across primes and operands)
@@ -551,7 +565,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": workload, "mode": "plain", "limbs_a": L, "limbs_b": L, "limb_bits": bits,
-                   "N": plan.N, "primes": plan.primes, "passes": plan.passes, "autotuned": tuned},
+                   "N": plan.N, "primes": plan.primes, "passes": plan.passes,
+                   "schedule": ["pass-major", "prime-major", "hybrid"][schedule], "autotuned": tuned},
         "latency_ms": {"p50": p50, "p99": p99},
         "bit_exact_vs_gmp": exact,
         "kernels": [{"name": k, "launches": v[2], "ms_per_product": v[0], "alg_bytes": v[1],

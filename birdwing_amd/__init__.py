@@ -33,6 +33,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbirdwing_hip.so")
 
 SET_DEFAULT, SET_U32_REF, SET_U32_WIDE, SET_U64_REF, SET_HYBRID, SET_D64 = -1, 0, 1, 2, 3, 4
+SCHED_PASS_MAJOR, SCHED_PRIME_MAJOR, SCHED_HYBRID = 0, 1, 2
 BW_OK, BW_ERR_ARG, BW_ERR_UNSUPPORTED_SIZE, BW_ERR_HIP, BW_ERR_NOMEM, BW_ERR_STATE = 0, -1, -2, -3, -4, -5
 
 
@@ -47,7 +48,7 @@ class PlanInfo(C.Structure):
                 ("primes", C.c_uint64 * 5), ("L_A", C.c_uint64), ("L_B", C.c_uint64),
                 ("N", C.c_uint64), ("N_exec", C.c_uint64), ("npass", C.c_int),
                 ("pass_log", C.c_int * 4), ("nlaunch", C.c_int), ("device_bytes", C.c_uint64),
-                ("digit_bits", C.c_int)]
+                ("digit_bits", C.c_int), ("schedule", C.c_int)]
 
 
 class BatchInfo(C.Structure):
@@ -285,8 +286,13 @@ class Plan:
         self.passes = [int(info.pass_log[i]) for i in range(info.npass)]
 
     def set_schedule(self, schedule):
-        """0: one launch per pass over all primes; 1: per-prime chains (default)."""
+        """SCHED_PASS_MAJOR: one launch per pass over all primes; SCHED_PRIME_MAJOR:
+        per-prime chains; SCHED_HYBRID: forward passes pass-major, fused and
+        inverse passes per prime."""
         _check(lib().bw_plan_set_schedule(self._p, schedule))
+        info = PlanInfo()
+        _check(lib().bw_plan_get_info(self._p, C.byref(info)))
+        self.info = info
 
     def status(self):
         _check(lib().bw_plan_status(self._p))

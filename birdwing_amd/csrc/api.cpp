@@ -225,6 +225,7 @@ extern "C" int bw_plan_create_ex(bw_plan** out, int limb_bits, uint64_t L_A, uin
         if (tune_lookup(TuneKey{set_id, logN, 0}, &c)) {   // autotuned split for this size
             p->passes = c.passes;
             p->gshift = c.gshift;
+            p->schedule = c.schedule;
         } else {
             default_passes(logN, p->passes);
         }
@@ -300,6 +301,7 @@ extern "C" int bw_plan_get_info(const bw_plan* p, bw_plan_info* info) {
     info->nlaunch = count_launches(p);
     info->device_bytes = p->dev_bytes;
     info->digit_bits = p->ingress == 2 ? 64 : p->limb_bits;
+    info->schedule = p->schedule;
     return BW_OK;
 }
 
@@ -427,6 +429,7 @@ extern "C" int bw_plan_tune(bw_plan* p, const void* d_a, const void* d_b, void* 
         TuneChoice cur;
         cur.passes = p->passes;
         cur.gshift = p->gshift;
+        cur.schedule = p->schedule;
         cands.insert(cands.begin(), cur);
     }
     hipEvent_t e0, e1;
@@ -434,30 +437,42 @@ extern "C" int bw_plan_tune(bw_plan* p, const void* d_a, const void* d_b, void* 
     HIP_TRY(hipEventCreate(&e1));
     TuneChoice best;
     best.ms = -1.0f;
-    for (const auto& c : cands) {
+    auto consider = [&](const TuneChoice& c) -> int {
         bool reg2 = true;
         for (int t : c.passes) reg2 = reg2 && t <= BW_REG2_MAX_LOG;
-        if (!reg2 && p->ingress == 2) continue;
-        if (!reg2 && c.passes != p->passes) continue;
+        if (!reg2 && p->ingress == 2) return BW_OK;
+        if (!reg2 && c.passes != p->passes) return BW_OK;
         Xform x = plan_xform(p);
         x.passes = c.passes;
         x.gshift = c.gshift;
+        x.schedule = c.schedule;
         float ms_min = -1.0f;
-        for (int r = 0; r < reps + 1 && !rc; r++) {
-            if (hipEventRecord(e0, st) != hipSuccess) { rc = fail(BW_ERR_HIP, "hipEventRecord"); break; }
-            if ((rc = run_transforms(x, d_a, d_b, p->in_A, p->in_B, st, nullptr))) break;
-            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-                rc = fail(BW_ERR_HIP, "tuning event");
-                break;
-            }
+        for (int r = 0; r < reps + 1; r++) {
+            if (hipEventRecord(e0, st) != hipSuccess) return fail(BW_ERR_HIP, "hipEventRecord");
+            int rc2 = run_transforms(x, d_a, d_b, p->in_A, p->in_B, st, nullptr);
+            if (rc2) return rc2;
+            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+                return fail(BW_ERR_HIP, "tuning event");
             float ms = 0.0f;
             (void)hipEventElapsedTime(&ms, e0, e1);
             if (r > 0 && (ms_min < 0 || ms < ms_min)) ms_min = ms;    // first run warms up
         }
-        if (rc) break;
         if (best.ms < 0 || ms_min < best.ms) {
             best = c;
             best.ms = ms_min;
+        }
+        return BW_OK;
+    };
+    // pass splits and tile caps first, then the launch order of the winner
+    for (const auto& c : cands)
+        if ((rc = consider(c))) break;
+    if (!rc && best.ms >= 0 && best.passes.size() >= 2) {
+        const TuneChoice w = best;
+        for (int sched : {BW_SCHED_PASS_MAJOR, BW_SCHED_PRIME_MAJOR, BW_SCHED_HYBRID}) {
+            if (sched == w.schedule) continue;
+            TuneChoice c = w;
+            c.schedule = sched;
+            if ((rc = consider(c))) break;
         }
     }
     (void)hipEventDestroy(e0);
@@ -466,6 +481,7 @@ extern "C" int bw_plan_tune(bw_plan* p, const void* d_a, const void* d_b, void* 
     if (best.ms >= 0) {
         p->passes = best.passes;
         p->gshift = best.gshift;
+        p->schedule = best.schedule;
         tune_store(TuneKey{p->set_id, p->logN, 0}, best);
     }
     return BW_OK;
@@ -474,7 +490,7 @@ extern "C" int bw_plan_tune(bw_plan* p, const void* d_a, const void* d_b, void* 
 extern "C" int bw_plan_set_schedule(bw_plan* p, int schedule) {
     int rc = check_plan(p);
     if (rc) return rc;
-    if (schedule != BW_SCHED_PASS_MAJOR && schedule != BW_SCHED_PRIME_MAJOR)
+    if (schedule != BW_SCHED_PASS_MAJOR && schedule != BW_SCHED_PRIME_MAJOR && schedule != BW_SCHED_HYBRID)
         return fail(BW_ERR_ARG, "unknown schedule %d", schedule);
     p->schedule = schedule;
     return BW_OK;

@@ -63,6 +63,7 @@ struct TuneKey {
 struct TuneChoice {
     std::vector<int> passes;
     int gshift = 0;
+    int schedule = 0;   // BW_SCHED_* (single-product plans)
     float ms = 0.0f;
 };
 bool tune_lookup(const TuneKey& k, TuneChoice* c);

@@ -127,7 +127,9 @@ hipError_t launch_pass(const Xform& x, int mode, const PassPlan& g, const PassPa
 int count_transform_launches(const Xform& x) {
     const int P = (int)x.passes.size();
     if (P == 1) return 1;
-    if (x.schedule == BW_SCHED_PRIME_MAJOR) return x.set->k * (2 * (P - 1) + 1 + (P - 1));
+    const int k = x.set->k;
+    if (x.schedule == BW_SCHED_PRIME_MAJOR) return k * (2 * (P - 1) + 1 + (P - 1));
+    if (x.schedule == BW_SCHED_HYBRID) return (P - 1) + k * (1 + (P - 1));
     return (P - 1) + 1 + (P - 1);
 }
 
@@ -141,6 +143,10 @@ int count_transform_launches(const Xform& x) {
 // B_p, the fused pass, the inverse passes -- runs before the next prime
 // starts, so each intermediate plane is re-read while the 256 MB Infinity
 // Cache still holds it.
+// BW_SCHED_HYBRID: forward passes pass-major (every prime and both operands in
+// one launch), then the fused pass and the inverse passes prime by prime: the
+// product plane of one prime (written by the fused pass) is re-read by the
+// inverse passes from the Infinity Cache.
 int run_transforms(const Xform& x, const void* d_a, const void* d_b, int64_t la, int64_t lb, hipStream_t st,
                    LaunchMarker* mk) {
     const int P = (int)x.passes.size();
@@ -160,39 +166,38 @@ int run_transforms(const Xform& x, const void* d_a, const void* d_b, int64_t la,
     }
     // stacked ingress names the operand by launch slot (A in 0, B in 1)
     if (x.bdesc && x.schedule == BW_SCHED_PRIME_MAJOR)
-        return fail(BW_ERR_ARG, "stacked ingress needs the pass-major schedule");
+        return fail(BW_ERR_ARG, "stacked ingress needs a pass-major forward schedule");
     const bool folded = pass_geom(x, P - 1).logT <= BW_REG2_MAX_LOG && pass_geom(x, P - 2).logT <= BW_REG2_MAX_LOG;
-    const bool pm = x.schedule == BW_SCHED_PRIME_MAJOR;
     const int k = x.set->k;
-    const int nchunk = pm ? k : 1;          // prime chunks
-    const int nper = pm ? 1 : k;            // primes per launch
     const void* raw[2] = {d_a, d_b};
     void* buf[2] = {x.bufA, x.bufB};
     const int64_t len[2] = {la, lb};
-    for (int c = 0; c < nchunk; c++) {
-        const int prime0 = pm ? c : 0;
-        // forward passes 1 .. P-1: per operand (prime-major) or both operands at once
-        for (int op = 0; op < (pm ? 2 : 1); op++) {
-            for (int j = 0; j < P - 1; j++) {
-                PassPlan g = pass_geom(x, j);
-                PassParams prm = base_params(x, g);
-                prm.prime0 = prime0;
-                prm.canon_out = 0;
-                const int nops = pm ? 1 : 2;
-                for (int q = 0; q < nops; q++) {
-                    const int which = pm ? op : q;
-                    prm.in[q] = j == 0 ? raw[which] : (const void*)buf[which];
-                    prm.in_len[q] = len[which];
-                    prm.out[q] = buf[which];
-                }
-                if (j == 0) prm.bdesc = x.bdesc;
-                prm.ingress = j == 0 ? x.ingress : 0;
-                prm.nops = nops;
-                PassLaunch L = {BW_MODE_FWD, g.logT, g.G, x.limb_bits, nper, nops, g.tiles};
-                LAUNCH_TRY(bw_launch_pass(L, prm, st));
-                MARK();
+    // forward passes 1 .. P-1 of primes [prime0, prime0 + nper): one operand
+    // (op 0 or 1) or both at once (op < 0)
+    auto forward = [&](int prime0, int nper, int op) -> int {
+        for (int j = 0; j < P - 1; j++) {
+            PassPlan g = pass_geom(x, j);
+            PassParams prm = base_params(x, g);
+            prm.prime0 = prime0;
+            prm.canon_out = 0;
+            const int nops = op < 0 ? 2 : 1;
+            for (int q = 0; q < nops; q++) {
+                const int which = op < 0 ? q : op;
+                prm.in[q] = j == 0 ? raw[which] : (const void*)buf[which];
+                prm.in_len[q] = len[which];
+                prm.out[q] = buf[which];
             }
+            if (j == 0) prm.bdesc = x.bdesc;
+            prm.ingress = j == 0 ? x.ingress : 0;
+            prm.nops = nops;
+            PassLaunch L = {BW_MODE_FWD, g.logT, g.G, x.limb_bits, nper, nops, g.tiles};
+            LAUNCH_TRY(bw_launch_pass(L, prm, st));
+            MARK();
         }
+        return BW_OK;
+    };
+    // fused pass and inverse passes of primes [prime0, prime0 + nper)
+    auto backward = [&](int prime0, int nper) -> int {
         {
             PassPlan g = pass_geom(x, P - 1);
             PassParams prm = base_params(x, g);
@@ -218,6 +223,22 @@ int run_transforms(const Xform& x, const void* d_a, const void* d_b, int64_t la,
             LAUNCH_TRY(bw_launch_pass(L, prm, st));
             MARK();
         }
+        return BW_OK;
+    };
+    int rc;
+    if (x.schedule == BW_SCHED_PRIME_MAJOR) {
+        for (int c = 0; c < k; c++) {
+            for (int op = 0; op < 2; op++)
+                if ((rc = forward(c, 1, op))) return rc;
+            if ((rc = backward(c, 1))) return rc;
+        }
+    } else if (x.schedule == BW_SCHED_HYBRID) {
+        if ((rc = forward(0, k, -1))) return rc;
+        for (int c = 0; c < k; c++)
+            if ((rc = backward(c, 1))) return rc;
+    } else {
+        if ((rc = forward(0, k, -1))) return rc;
+        if ((rc = backward(0, k))) return rc;
     }
     return BW_OK;
 }

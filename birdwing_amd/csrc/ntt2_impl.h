@@ -177,13 +177,20 @@ struct ArU64x {
     __device__ static const PR& prime(const PC32& c) { return c.P; }
 };
 
-// Codelets with roots up to order 32 (R <= 32).
-template <class AR, int RLOG>
+// Codelets with roots up to order 32 (R <= 32).  HZ: the upper half of the
+// input is zero (a zero-padded operand), so the first level reduces to copies
+// and twiddle products and x[R/2 ..] is never read.
+template <class AR, int RLOG, bool HZ = false>
 __device__ __forceinline__ void dft32_dif(typename AR::V* x, const typename AR::PC32& pc) {
     constexpr int R = 1 << RLOG;
 #pragma unroll
     for (int q = 0; q < RLOG; q++) {
         const int h = R >> (q + 1);
+        if (HZ && q == 0) {
+#pragma unroll
+            for (int i = 0; i < h; i++) x[i + h] = (i == 0) ? x[i] : AR::mulc(x[i], pc.w32[i * (16 / h)], pc.P);
+            continue;
+        }
 #pragma unroll
         for (int blk = 0; blk < R; blk += 2 * h) {
 #pragma unroll
@@ -313,7 +320,7 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     // (loads without per-element tests, 64-bit when the limb pair is 8-byte
     // aligned), every digit past the end (no loads), and digits straddling the
     // end (per-element tests).
-    auto load0 = [&](int which, IW* w) {
+    auto load0 = [&](int which, IW* w) -> bool {
         int64_t base = line_base(g0) + j0 * tstride;
         const int64_t step = (int64_t)R1 * tstride;
         if constexpr (INGRESS != 0) {
@@ -322,6 +329,7 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
             ingress_src(which, raw, len, base);
             constexpr int LW = INGRESS == 2 ? 2 : 1;          // limbs per digit
             const int64_t last = base + (int64_t)(R0 - 1) * step;
+            const bool upper_zero = R0 > 1 && LW * (base + (int64_t)(R0 / 2) * step) >= len;
             if (LW * last + (LW - 1) < len) {
                 if constexpr (INGRESS == 2) {
                     if ((reinterpret_cast<uintptr_t>(raw) & 7) == 0) {
@@ -357,20 +365,35 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
                     }
                 }
             }
+            return upper_zero;
         } else {
             const W* in = reinterpret_cast<const W*>(prm.in[which]) + plane_off;
 #pragma unroll
             for (int m = 0; m < R0; m++) w[m] = in[base + m * step];
+            return false;
         }
     };
-    auto fwd_stage0_from = [&](const IW* w) {
+    // hz: the upper half of the group is zero padding (raw operands only)
+    auto fwd_stage0_from = [&](const IW* w, bool hz) {
         V x[R0];
+        if constexpr (INGRESS != 0 && R0 > 1) {
+            if (hz) {
 #pragma unroll
-        for (int m = 0; m < R0; m++) {
-            if constexpr (INGRESS != 0) x[m] = digit_value(w[m]);
-            else x[m] = AR::from_word(w[m], P);
+                for (int m = 0; m < R0 / 2; m++) x[m] = digit_value(w[m]);
+                dft32_dif<AR, R0LOG, true>(x, pc);
+            } else {
+#pragma unroll
+                for (int m = 0; m < R0; m++) x[m] = digit_value(w[m]);
+                dft32_dif<AR, R0LOG>(x, pc);
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < R0; m++) {
+                if constexpr (INGRESS != 0) x[m] = digit_value(w[m]);
+                else x[m] = AR::from_word(w[m], P);
+            }
+            dft32_dif<AR, R0LOG>(x, pc);
         }
-        dft32_dif<AR, R0LOG>(x, pc);
 #pragma unroll
         for (int k = 1; k < R0; k++) x[brev(k, R0LOG)] = AR::mul(x[brev(k, R0LOG)], twl[j0 * k], P);
 #pragma unroll
@@ -380,8 +403,8 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     };
     auto fwd_stage0 = [&](int which) {
         IW w[R0];
-        load0(which, w);
-        fwd_stage0_from(w);
+        const bool hz = load0(which, w);
+        fwd_stage0_from(w, hz);
     };
     // fused pass: A's stage-1 spectrum, then the pointwise product, held in registers
     constexpr int KEEPN = IT1 * R1;
@@ -502,10 +525,10 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     };
     if constexpr (MODE == BW_MODE_FWD) {
         IW w[R0];
-        load0(op, w);
+        const bool hz = load0(op, w);
         fill_twiddles();
         __syncthreads();
-        fwd_stage0_from(w);
+        fwd_stage0_from(w, hz);
         __syncthreads();
         fwd_stage1(op, std::integral_constant<int, 0>());
     } else if constexpr (MODE == BW_MODE_INV) {
@@ -518,24 +541,25 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         // register sets would cost occupancy (radix-32 stage 0)
         constexpr bool PREFETCH = BW_FUSED_PREFETCH && R0 <= 16;
         IW wb[PREFETCH ? R0 : 1];
+        bool hzb = false;
         if constexpr (PREFETCH) {
             IW wa[R0];
-            load0(0, wa);
-            load0(1, wb);
+            const bool hza = load0(0, wa);
+            hzb = load0(1, wb);
             fill_twiddles();
             __syncthreads();
-            fwd_stage0_from(wa);
+            fwd_stage0_from(wa, hza);
         } else {
             IW wa[R0];
-            load0(0, wa);
+            const bool hza = load0(0, wa);
             fill_twiddles();
             __syncthreads();
-            fwd_stage0_from(wa);
+            fwd_stage0_from(wa, hza);
         }
         __syncthreads();
         fwd_stage1(0, std::integral_constant<int, 1>());
         __syncthreads();
-        if constexpr (PREFETCH) fwd_stage0_from(wb);
+        if constexpr (PREFETCH) fwd_stage0_from(wb, hzb);
         else fwd_stage0(1);
         __syncthreads();
         // B's spectrum multiplies into A's

@@ -56,7 +56,7 @@ enum {
 };
 
 /* Launch order of a plan's passes (bw_plan_set_schedule). */
-enum { BW_SCHED_PASS_MAJOR = 0, BW_SCHED_PRIME_MAJOR = 1 };
+enum { BW_SCHED_PASS_MAJOR = 0, BW_SCHED_PRIME_MAJOR = 1, BW_SCHED_HYBRID = 2 };
 
 enum { BW_SET_DEFAULT = -1, BW_SET_U32_REF = 0, BW_SET_U32_WIDE = 1, BW_SET_U64_REF = 2, BW_SET_HYBRID = 3,
        BW_SET_D64 = 4 };
@@ -76,12 +76,13 @@ typedef struct bw_plan_info {
     int nlaunch;           /* kernel launches per bw_execute */
     uint64_t device_bytes; /* device memory held by the plan */
     int digit_bits;        /* 64 for the five-prime 64-bit-digit set, else limb_bits */
+    int schedule;          /* launch order, BW_SCHED_* */
 } bw_plan_info;
 
 /* ---- library ---------------------------------------------------------- */
 /* Layout version of the structs in this header; a caller built against one
  * version must not run against a library reporting another. */
-#define BW_ABI_VERSION 2
+#define BW_ABI_VERSION 3
 int bw_abi_version(void);
 const char* bw_last_error(void);
 const char* bw_version(void);
@@ -122,9 +123,11 @@ int bw_execute(bw_plan* plan, const void* d_a, const void* d_b, void* d_c, void*
  * 64-bit path: c0 = lo, c1 = mid, c2 = hi 32 bits (U160). */
 int bw_execute_crt(bw_plan* plan, const void* d_a, const void* d_b, uint64_t* d_c0, uint64_t* d_c1,
                    uint32_t* d_c2, void* stream);
-/* Launch order: BW_SCHED_PASS_MAJOR (default) issues one launch per pass over
- * all primes; BW_SCHED_PRIME_MAJOR finishes one prime's chain of passes before
- * the next so intermediates are re-read from the Infinity Cache. */
+/* Launch order: BW_SCHED_PASS_MAJOR issues one launch per pass over all
+ * primes; BW_SCHED_PRIME_MAJOR finishes one prime's chain of passes before the
+ * next so intermediates are re-read from the Infinity Cache; BW_SCHED_HYBRID
+ * runs the forward passes pass-major and the fused and inverse passes prime by
+ * prime.  Plans start pass-major; bw_plan_tune also chooses the schedule. */
 int bw_plan_set_schedule(bw_plan* plan, int schedule);
 /* Autotuner: times candidate pass splits and tile caps of this plan's
  * transforms on the given operands (reps timed runs each, synchronizing),

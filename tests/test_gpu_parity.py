@@ -336,10 +336,10 @@ def test_d64_pass_splits(gpu, oracle, passes):
     _product_check(gpu, oracle, A, B, 64, gpu.SET_D64, passes)
 
 
-@pytest.mark.parametrize("schedule", [0, 1])
+@pytest.mark.parametrize("schedule", [0, 1, 2])
 @pytest.mark.parametrize("bits,L", [(32, 3000), (32, 1 << 18), (64, 1 << 16)])
 def test_both_schedules(gpu, oracle, bits, L, schedule):
-    """Pass-major and prime-major launch orders give the same product."""
+    """Pass-major, prime-major and hybrid launch orders give the same product."""
     d = dev()
     A = oracle.uniform_limbs(L, 5, bits)
     B = oracle.uniform_limbs(L - 7, 6, bits)
