@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <chrono>
 #include <vector>
 #include <hip/hip_runtime_api.h>
 
@@ -76,7 +77,9 @@ struct NTTContext {
     uint64_t* d_C_lo = nullptr;
 #endif
     OutputLimbType* d_out = nullptr;
-    bw_plan* plan = nullptr;
+    bw_plan* plan = nullptr;        // products: the library's automatic prime set
+    bw_plan* crt_plan = nullptr;    // CRT early exit: the precompute's three-prime set (made on first use)
+    int prime_set = 0;
 };
 
 namespace bw_dropin {
@@ -92,7 +95,16 @@ inline void hip_check(hipError_t e, const char* what) {
 }
 }  // namespace bw_dropin
 
+namespace bw_dropin {
+inline double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+}  // namespace bw_dropin
+
+// Host side of the precompute: the prime set and its Garner constants (the
+// twiddle tables are built once per prime set by upload_ntt_precomputed).
 inline NTTPrecomputed precompute_ntt(size_t N, PrecomputeTiming* timing_out = nullptr) {
+    const double t0 = bw_dropin::now_ms();
     ensure_ntt_size_supported(N);
     NTTPrecomputed pre;
     pre.N = N;
@@ -102,22 +114,36 @@ inline NTTPrecomputed precompute_ntt(size_t N, PrecomputeTiming* timing_out = nu
     uint64_t P[3] = {0, 0, 0};
     if (bw_prime_set_info(pre.prime_set, &k, P, nullptr) != BW_OK) bw_dropin::die("precompute_ntt");
     pre.primes.assign(P, P + k);
+    const double t1 = bw_dropin::now_ms();
     if (k == NUM_MODULI) {
         std::vector<TestDataTypeUint> pv(P, P + k);
         pre.garner = compute_garner_params(pv);
     }
-    if (timing_out) *timing_out = PrecomputeTiming{};
+    const double t2 = bw_dropin::now_ms();
+    if (timing_out) {
+        *timing_out = PrecomputeTiming{};
+        timing_out->factors_ms = (float)(t1 - t0);
+        timing_out->garner_host_ms = (float)(t2 - t1);
+        timing_out->total_ms = (float)(t2 - t0);
+    }
     return pre;
 }
 
-// Twiddle tables are built and uploaded once per prime set and device.
+// Twiddle tables are built and uploaded once per prime set and device (the
+// first call pays for it; later ones find them resident).
 inline void upload_ntt_precomputed(NTTPrecomputed& pre, SetupUploadTiming* timing_out = nullptr) {
     if (pre.gpu_uploaded) return;
+    const double t0 = bw_dropin::now_ms();
     int dev = 0;
     bw_dropin::hip_check(hipGetDevice(&dev), "hipGetDevice");
     if (bw_init(dev) != BW_OK) bw_dropin::die("upload_ntt_precomputed");
+    bw_dropin::hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     pre.gpu_uploaded = true;
-    if (timing_out) *timing_out = SetupUploadTiming{};
+    if (timing_out) {
+        *timing_out = SetupUploadTiming{};
+        timing_out->twiddle_upload_ms = (float)(bw_dropin::now_ms() - t0);
+        timing_out->total_ms = timing_out->twiddle_upload_ms;
+    }
 }
 
 inline NTTContext allocate_ntt_context(const NTTPrecomputed& pre, size_t L_A, size_t L_B) {
@@ -130,7 +156,10 @@ inline NTTContext allocate_ntt_context(const NTTPrecomputed& pre, size_t L_A, si
     ctx.logN = pre.logN;
     ctx.L_A = L_A;
     ctx.L_B = L_B;
-    if (bw_plan_create(&ctx.plan, BW_CFG_LIMB_BITS, L_A, L_B, pre.prime_set) != BW_OK)
+    ctx.prime_set = pre.prime_set;
+    // Products are the same under every prime set; the automatic choice (the
+    // five-prime 64-bit-digit set where it fits) is the fastest.
+    if (bw_plan_create(&ctx.plan, BW_CFG_LIMB_BITS, L_A, L_B, -1) != BW_OK)
         bw_dropin::die("allocate_ntt_context");
     using bw_dropin::hip_check;
     hip_check(hipMalloc((void**)&ctx.a_raw_dev, L_A * sizeof(InputLimbType)), "hipMalloc");
@@ -148,13 +177,26 @@ inline void execute_ntt_multiply(NTTContext& ctx, const InputLimbType* a_pinned,
                                  std::vector<uint32_t>* crt_hi32_out = nullptr) {
     using bw_dropin::hip_check;
     hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+#if defined(NATIVE_HOST_LIMBS)
+    const bool early = crt_lo_out && crt_mid_out && crt_hi32_out;
+#else
+    const bool early = crt_hi_out && crt_lo_out;
+#endif
+    (void)crt_hi_out;
+    (void)crt_mid_out;
+    (void)crt_hi32_out;
+    // the CRT coefficients are those of the precompute's three-prime set
+    if (early && !ctx.crt_plan &&
+        bw_plan_create(&ctx.crt_plan, BW_CFG_LIMB_BITS, ctx.L_A, ctx.L_B, ctx.prime_set) != BW_OK)
+        bw_dropin::die("execute_ntt_multiply");
+    bw_plan* const run = early ? ctx.crt_plan : ctx.plan;
     if (timing_out) {
         hip_check(hipEventCreate(&e0), "hipEventCreate");
         hip_check(hipEventCreate(&e1), "hipEventCreate");
         hip_check(hipEventCreate(&e2), "hipEventCreate");
         hip_check(hipEventCreate(&e3), "hipEventCreate");
         hip_check(hipEventRecord(e0, ctx.stream_a), "hipEventRecord");
-        bw_plan_set_timing(ctx.plan, 1);
+        bw_plan_set_timing(run, 1);
     }
     hip_check(hipMemcpyAsync(ctx.a_raw_dev, a_pinned, ctx.L_A * sizeof(InputLimbType), hipMemcpyHostToDevice,
                              ctx.stream_a), "H2D");
@@ -163,13 +205,8 @@ inline void execute_ntt_multiply(NTTContext& ctx, const InputLimbType* a_pinned,
     if (timing_out) hip_check(hipEventRecord(e1, ctx.stream_a), "hipEventRecord");
 
     bw_plan_info info;
-    bw_plan_get_info(ctx.plan, &info);
+    bw_plan_get_info(run, &info);
     const size_t Nx = info.N_exec;
-#if defined(NATIVE_HOST_LIMBS)
-    const bool early = crt_lo_out && crt_mid_out && crt_hi32_out;
-#else
-    const bool early = crt_hi_out && crt_lo_out;
-#endif
     if (early) {
 #if defined(NATIVE_HOST_LIMBS)
         if (!ctx.d_C_lo) {
@@ -177,7 +214,7 @@ inline void execute_ntt_multiply(NTTContext& ctx, const InputLimbType* a_pinned,
             hip_check(hipMalloc((void**)&ctx.d_C_mid, Nx * 8), "hipMalloc");
             hip_check(hipMalloc((void**)&ctx.d_C_hi32, Nx * 4), "hipMalloc");
         }
-        if (bw_execute_crt(ctx.plan, ctx.a_raw_dev, ctx.b_raw_dev, ctx.d_C_lo, ctx.d_C_mid, ctx.d_C_hi32,
+        if (bw_execute_crt(run, ctx.a_raw_dev, ctx.b_raw_dev, ctx.d_C_lo, ctx.d_C_mid, ctx.d_C_hi32,
                            (void*)ctx.stream_a) != BW_OK)
             bw_dropin::die("execute_ntt_multiply");
         crt_lo_out->resize(ctx.N);
@@ -192,7 +229,7 @@ inline void execute_ntt_multiply(NTTContext& ctx, const InputLimbType* a_pinned,
             hip_check(hipMalloc((void**)&ctx.d_C_lo, Nx * 8), "hipMalloc");
             hip_check(hipMalloc((void**)&ctx.d_C_hi, Nx * 8), "hipMalloc");
         }
-        if (bw_execute_crt(ctx.plan, ctx.a_raw_dev, ctx.b_raw_dev, ctx.d_C_lo, ctx.d_C_hi, nullptr,
+        if (bw_execute_crt(run, ctx.a_raw_dev, ctx.b_raw_dev, ctx.d_C_lo, ctx.d_C_hi, nullptr,
                            (void*)ctx.stream_a) != BW_OK)
             bw_dropin::die("execute_ntt_multiply");
         crt_hi_out->resize(ctx.N);
@@ -219,21 +256,29 @@ inline void execute_ntt_multiply(NTTContext& ctx, const InputLimbType* a_pinned,
         hip_check(hipEventElapsedTime(&t.h2d_ms, e0, e1), "hipEventElapsedTime");
         float ms[64];
         int n = 0;
-        bw_plan_timing(ctx.plan, ms, 64, &n);
-        bw_plan_set_timing(ctx.plan, 0);
-        // launch order per prime chunk: forward passes (per operand when the
-        // chunk is one prime), the fused pass (last forward + pointwise + first
-        // inverse), the remaining inverse passes; then one Garner(+carry) launch
+        bw_plan_timing(run, ms, 64, &n);
+        bw_plan_set_timing(run, 0);
+        // launch order (bw_plan_info.schedule): forward passes, the fused pass
+        // (last forward + pointwise + first inverse), the remaining inverse
+        // passes -- over all primes at once (pass-major), prime by prime
+        // (prime-major, forward passes per operand), or forward over all primes
+        // and the rest prime by prime (hybrid); then one Garner(+carry) launch
         const int P = info.npass, k = info.k;
-        const bool per_prime = P > 1 && n - 1 == k * (4 * P - 3);
-        const int nfwd = P == 1 ? 0 : (per_prime ? 2 : 1) * (P - 1);
-        const int per = P == 1 ? 1 : nfwd + 1 + (P - 1);
         for (int i = 0; i + 1 < n; i++) {
-            const int pos = i % per;
-            if (pos < nfwd) t.fwd_pad_ntt_ms += ms[i];
-            else if (pos == nfwd) t.pointwise_mul_ms += ms[i];
-            else t.intt_ms += ms[i];
+            int kind = 1;                               // 0 forward, 1 fused, 2 inverse
+            if (P > 1) {
+                if (info.schedule == BW_SCHED_PRIME_MAJOR) {
+                    const int pos = i % (3 * (P - 1) + 1);
+                    kind = pos < 2 * (P - 1) ? 0 : (pos == 2 * (P - 1) ? 1 : 2);
+                } else if (info.schedule == BW_SCHED_HYBRID) {
+                    kind = i < P - 1 ? 0 : ((i - (P - 1)) % P == 0 ? 1 : 2);
+                } else {
+                    kind = i < P - 1 ? 0 : (i == P - 1 ? 1 : 2);
+                }
+            }
+            (kind == 0 ? t.fwd_pad_ntt_ms : kind == 1 ? t.pointwise_mul_ms : t.intt_ms) += ms[i];
         }
+        (void)k;
         if (n > 0) (early ? t.crt_ms : t.carry_ms) = ms[n - 1];
         t.fwd_pad_ntt_a_ms = t.fwd_pad_ntt_b_ms = t.fwd_pad_ntt_ms / 2;
         t.ingress_fwd_ms = t.h2d_ms + t.fwd_pad_ntt_ms;
@@ -246,7 +291,8 @@ inline void execute_ntt_multiply(NTTContext& ctx, const InputLimbType* a_pinned,
 
 inline void cleanup_ntt_context(NTTContext& ctx) {
     bw_plan_destroy(ctx.plan);
-    ctx.plan = nullptr;
+    bw_plan_destroy(ctx.crt_plan);
+    ctx.plan = ctx.crt_plan = nullptr;
     (void)hipFree(ctx.a_raw_dev);
     (void)hipFree(ctx.b_raw_dev);
     (void)hipFree(ctx.d_out);
