@@ -71,6 +71,8 @@ _vp, _u64, _i, _sz, _cp = C.c_void_p, C.c_uint64, C.c_int, C.c_size_t, C.c_char_
 _pi, _pu64 = C.POINTER(C.c_int), C.POINTER(C.c_uint64)
 # name -> (restype, argtypes); every symbol declared in include/birdwing.h
 SIGNATURES = {
+    "bw_tables_save": (_i, [_i, C.c_char_p]),
+    "bw_tables_load": (_i, [C.c_char_p, C.POINTER(C.c_int)]),
     "bw_abi_version": (_i, []),
     "bw_last_error": (_cp, []),
     "bw_version": (_cp, []),
@@ -175,6 +177,18 @@ def _stream(stream):
 # ---- size policy (src/ntt_limits.cpp) --------------------------------------
 def padded_ntt_size(L_A, L_B):
     return int(lib().bw_padded_ntt_size(L_A, L_B))
+
+
+def tables_save(prime_set, path):
+    """Write the host tables of a prime set (SET_*) to `path` (precompute cache)."""
+    _check(lib().bw_tables_save(prime_set, os.fsencode(path)))
+
+
+def tables_load(path):
+    """Load a saved table image; later table builds for its set use it. Returns the set."""
+    s = C.c_int(-1)
+    _check(lib().bw_tables_load(os.fsencode(path), C.byref(s)))
+    return int(s.value)
 
 
 def default_prime_set(limb_bits, N):

@@ -46,6 +46,9 @@ extern "C" int bw_init(int device) {
     return rc;
 }
 
+extern "C" int bw_tables_save(int set, const char* path) { return tables_save(set, path); }
+extern "C" int bw_tables_load(const char* path, int* set) { return tables_load(path, set); }
+
 // ---------------------------------------------------------------------------
 // size policy
 extern "C" uint64_t bw_padded_ntt_size(uint64_t L_A, uint64_t L_B) { return padded_ntt_size(L_A, L_B); }

@@ -46,7 +46,8 @@ __device__ __forceinline__ void garner_u64(uint64_t r0, uint64_t r1, uint64_t r2
     w2 = (uint64_t)((s1 >> 64) + (c >> 64));
 }
 
-// Five-prime Garner for 64-bit digits: mixed-radix digits
+// Five-prime Garner for 64-bit digits (residues canonical or centered, |r| < p):
+// mixed-radix digits
 //   k_j = (...((r_j - k_0) p_0^-1 - k_1) p_1^-1 ... - k_{j-1}) p_{j-1}^-1 mod p_j
 // in FP64 products, then x = k_0 + p_0 (k_1 + p_1 (k_2 + p_2 (k_3 + p_3 k_4)))
 // by Horner steps on 32-bit words; x < 2^154 -> three 64-bit words.
@@ -55,7 +56,7 @@ __device__ __forceinline__ void garner_d64(const uint32_t* r, int64_t plane, int
     uint32_t k[5];
 #pragma unroll
     for (int j = 0; j < 5; j++) {
-        double t = (double)r[j * plane + n];
+        double t = (double)(int32_t)r[j * plane + n];   // canonical or centered residue
 #pragma unroll
         for (int i = 0; i < j; i++) t = f64_mulmod(t - (double)k[i], g.inv[j][i][0], g.inv[j][i][1], g.fp[j]);
         k[j] = f64_canon(t, g.fp[j]);

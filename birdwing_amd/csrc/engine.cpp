@@ -29,9 +29,43 @@ int fail(int code, const char* fmt, ...) {
 const char* last_error() { return g_err.c_str(); }
 
 // ---------------------------------------------------------------------------
-// per (device, set) tables
+// per (device, set) tables, built from one host image per set (made here, or
+// loaded from a saved image)
 static std::mutex g_tab_mu;
 static std::map<std::pair<int, int>, std::unique_ptr<DevTables>> g_tabs;
+static std::map<int, HostTables> g_host;     // guarded by g_tab_mu
+
+static const HostTables& host_tables_locked(const PrimeSet& s) {
+    auto it = g_host.find(s.id);
+    if (it == g_host.end()) {
+        HostTables h;
+        build_tables(s, h);
+        it = g_host.emplace(s.id, std::move(h)).first;
+    }
+    return it->second;
+}
+
+int tables_save(int set_id, const char* path) {
+    const PrimeSet* s = prime_set(set_id);
+    if (!s) return fail(BW_ERR_ARG, "unknown prime set %d", set_id);
+    if (!path) return fail(BW_ERR_ARG, "NULL path");
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    std::string why;
+    if (!write_table_image(path, set_id, host_tables_locked(*s), &why)) return fail(BW_ERR_ARG, "%s", why.c_str());
+    return BW_OK;
+}
+
+int tables_load(const char* path, int* set_id) {
+    if (!path) return fail(BW_ERR_ARG, "NULL path");
+    HostTables h;
+    int id = -1;
+    std::string why;
+    if (!read_table_image(path, &id, h, &why)) return fail(BW_ERR_ARG, "table image %s: %s", path, why.c_str());
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    g_host[id] = std::move(h);
+    if (set_id) *set_id = id;
+    return BW_OK;
+}
 
 int get_tables(int set_id, DevTables** out) {
     int dev = 0;
@@ -42,14 +76,13 @@ int get_tables(int set_id, DevTables** out) {
     if (it != g_tabs.end()) { *out = it->second.get(); return BW_OK; }
     const PrimeSet* s = prime_set(set_id);
     if (!s) return fail(BW_ERR_ARG, "unknown prime set %d", set_id);
-    HostTables h;
-    build_tables(*s, h);
+    const HostTables& h = host_tables_locked(*s);
     std::unique_ptr<DevTables> d(new DevTables());
     d->t.limb_bits = s->limb_bits;
     d->t.k = s->k;
     d->t.table_log = h.table_log;
     void** ptrs[4] = {&d->t.consts, &d->t.tw_stage, &d->t.tw_lo, &d->t.tw_hi};
-    std::vector<unsigned char>* srcs[4] = {&h.consts, &h.tw_stage, &h.tw_lo, &h.tw_hi};
+    const std::vector<unsigned char>* srcs[4] = {&h.consts, &h.tw_stage, &h.tw_lo, &h.tw_hi};
     for (int i = 0; i < 4; i++) {
         HIP_TRY(hipMalloc(ptrs[i], srcs[i]->size()));
         HIP_TRY(hipMemcpy(*ptrs[i], srcs[i]->data(), srcs[i]->size(), hipMemcpyHostToDevice));
@@ -150,9 +183,13 @@ int count_transform_launches(const Xform& x) {
 int run_transforms(const Xform& x, const void* d_a, const void* d_b, int64_t la, int64_t lb, hipStream_t st,
                    LaunchMarker* mk) {
     const int P = (int)x.passes.size();
+    // the final inverse pass stores canonical residues for the three-prime
+    // Garner kernels; the five-prime one reads centered residues as they are
+    const int final_canon = x.ingress == 2 ? 0 : 1;
     if (P == 1) {
         PassPlan g = pass_geom(x, 0);
         PassParams prm = base_params(x, g);
+        prm.canon_out = final_canon;
         prm.ingress = x.ingress;
         prm.in[0] = d_a;
         prm.in[1] = d_b;
@@ -217,7 +254,7 @@ int run_transforms(const Xform& x, const void* d_a, const void* d_b, int64_t la,
             prm.prime0 = prime0;
             prm.in[0] = x.bufA;
             prm.out[0] = x.bufA;
-            prm.canon_out = j == 0 ? 1 : 0;
+            prm.canon_out = j == 0 ? final_canon : 0;
             prm.fold_ninv = (j == P - 2) && folded;
             PassLaunch L = {BW_MODE_INV, g.logT, g.G, x.limb_bits, nper, 1, g.tiles};
             LAUNCH_TRY(bw_launch_pass(L, prm, st));

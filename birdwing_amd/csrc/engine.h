@@ -24,6 +24,9 @@ struct DevTables {
 };
 // Tables of prime set `set_id` on the current device (built on first use).
 int get_tables(int set_id, DevTables** out);
+// Host images of a prime set's tables saved to / loaded from a file (host only).
+int tables_save(int set_id, const char* path);
+int tables_load(const char* path, int* set_id);
 
 // One transform job: B products of one padded size N = 2^logN stacked along
 // the outer dimension.  Prime plane i of bufA / bufB holds product b at

@@ -1,5 +1,6 @@
 // host_math.cpp -- host-side number theory for plans (see host_math.h).
 #include "host_math.h"
+#include <cstdio>
 #include <cstring>
 
 namespace bw {
@@ -310,6 +311,94 @@ void ninv_pairs(const PrimeSet& s, uint64_t N, std::vector<uint64_t>& out64, std
         out64[2 * i] = mulmod(mulmod(ni, r, p), r, p);
         out64[2 * i + 1] = 0;
     }
+}
+
+// ---------------------------------------------------------------------------
+// table images (precompute cache)
+static const char kImageMagic[8] = {'B', 'W', 'T', 'A', 'B', 'L', 'E', 'S'};
+static const uint32_t kImageVersion = 1;
+
+static uint64_t fnv1a(uint64_t h, const unsigned char* p, size_t n) {
+    for (size_t i = 0; i < n; i++) h = (h ^ p[i]) * 0x100000001b3ULL;
+    return h;
+}
+
+namespace {
+struct ImageHeader {
+    char magic[8];
+    uint32_t version;
+    int32_t set_id, k, table_log;
+    uint64_t primes[BW_MAX_PRIMES];
+    uint64_t sizes[4];
+};
+}  // namespace
+
+bool write_table_image(const char* path, int set_id, const HostTables& h, std::string* why) {
+    const PrimeSet* s = prime_set(set_id);
+    if (!s) { if (why) *why = "unknown prime set"; return false; }
+    ImageHeader hd;
+    std::memset(&hd, 0, sizeof(hd));
+    std::memcpy(hd.magic, kImageMagic, 8);
+    hd.version = kImageVersion;
+    hd.set_id = set_id;
+    hd.k = s->k;
+    hd.table_log = h.table_log;
+    for (int i = 0; i < s->k; i++) hd.primes[i] = s->p[i];
+    const std::vector<unsigned char>* blobs[4] = {&h.consts, &h.tw_stage, &h.tw_lo, &h.tw_hi};
+    for (int i = 0; i < 4; i++) hd.sizes[i] = blobs[i]->size();
+    FILE* f = std::fopen(path, "wb");
+    if (!f) { if (why) *why = std::string("cannot open ") + path + " for writing"; return false; }
+    uint64_t sum = fnv1a(0xcbf29ce484222325ULL, reinterpret_cast<const unsigned char*>(&hd), sizeof(hd));
+    bool ok = std::fwrite(&hd, sizeof(hd), 1, f) == 1;
+    for (int i = 0; i < 4 && ok; i++) {
+        sum = fnv1a(sum, blobs[i]->data(), blobs[i]->size());
+        ok = blobs[i]->empty() || std::fwrite(blobs[i]->data(), blobs[i]->size(), 1, f) == 1;
+    }
+    ok = ok && std::fwrite(&sum, sizeof(sum), 1, f) == 1;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok && why) *why = std::string("write to ") + path + " failed";
+    return ok;
+}
+
+bool read_table_image(const char* path, int* set_id, HostTables& h, std::string* why) {
+    auto bad = [&](const std::string& m) { if (why) *why = m; return false; };
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return bad(std::string("cannot open ") + path);
+    ImageHeader hd;
+    if (std::fread(&hd, sizeof(hd), 1, f) != 1) { std::fclose(f); return bad("truncated header"); }
+    if (std::memcmp(hd.magic, kImageMagic, 8) != 0 || hd.version != kImageVersion) {
+        std::fclose(f);
+        return bad("not a table image of this format");
+    }
+    const PrimeSet* s = prime_set(hd.set_id);
+    bool same = s && hd.k == s->k && hd.table_log == set_max_log(*s);
+    for (int i = 0; same && i < BW_MAX_PRIMES; i++) same = hd.primes[i] == (i < s->k ? s->p[i] : 0);
+    if (same) {   // the sizes build_tables produces for this set
+        const uint64_t k = (uint64_t)s->k;
+        const uint64_t want[4] = {k * (s->limb_bits == 32 ? sizeof(F64PrimeDev) : sizeof(U64PrimeDev)),
+                                  k * ((uint64_t)1 << BW_STAGE_LOG) * 16, k * ((uint64_t)1 << BW_TW_LO_LOG) * 16,
+                                  k * ((uint64_t)1 << (hd.table_log - BW_TW_LO_LOG)) * 16};
+        for (int i = 0; i < 4; i++) same = same && hd.sizes[i] == want[i];
+    }
+    if (!same) { std::fclose(f); return bad("image does not match this library's prime set"); }
+    std::vector<unsigned char>* blobs[4] = {&h.consts, &h.tw_stage, &h.tw_lo, &h.tw_hi};
+    uint64_t sum = fnv1a(0xcbf29ce484222325ULL, reinterpret_cast<const unsigned char*>(&hd), sizeof(hd));
+    for (int i = 0; i < 4; i++) {
+        blobs[i]->resize(hd.sizes[i]);
+        if (hd.sizes[i] && std::fread(blobs[i]->data(), hd.sizes[i], 1, f) != 1) {
+            std::fclose(f);
+            return bad("truncated tables");
+        }
+        sum = fnv1a(sum, blobs[i]->data(), blobs[i]->size());
+    }
+    uint64_t stored = 0;
+    const bool have = std::fread(&stored, sizeof(stored), 1, f) == 1;
+    const bool extra = std::fgetc(f) != EOF;
+    std::fclose(f);
+    if (!have || extra || stored != sum) return bad("checksum mismatch (damaged image)");
+    h.table_log = hd.table_log;
+    *set_id = hd.set_id;
+    return true;
 }
 
 }  // namespace bw

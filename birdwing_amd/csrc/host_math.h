@@ -56,6 +56,13 @@ struct HostTables {
     std::vector<unsigned char> tw_hi;     // k * 2^(L-12) pairs
 };
 void build_tables(const PrimeSet& s, HostTables& out);
+// Serialised host tables of one prime set (a precompute cache on disk):
+// magic, format version, set id, k, table_log, the primes, the four blob
+// sizes, the blobs, and an FNV-1a 64 checksum of everything before it.
+// read_table_image rejects files of another format, of a prime set whose
+// primes or sizes differ from this library's, and damaged files.
+bool write_table_image(const char* path, int set_id, const HostTables& h, std::string* why);
+bool read_table_image(const char* path, int* set_id, HostTables& h, std::string* why);
 GarnerDev garner_constants(const PrimeSet& s);
 GarnerD64Dev garner_d64_constants(const PrimeSet& s);
 void f64_prime(uint64_t p, F64Prime& out);

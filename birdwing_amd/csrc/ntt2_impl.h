@@ -465,7 +465,8 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
             }
         }
     };
-    // stage 1 inverse: keep (fused) or HBM with the inverse pass twiddle -> inverse DFT_R1 -> LDS
+    // stage 1 inverse: keep (fused) or HBM with the inverse pass twiddle -> inverse DFT_R1
+    // -> inverse stage twiddle -> LDS
     auto inv_stage1 = [&](auto km) {
         constexpr bool KEEP = decltype(km)::value != 0;
 #pragma unroll
@@ -496,19 +497,23 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
                 }
             }
             dft32_dit_inv<AR, R1LOG>(x, pc);
+            // the inverse stage twiddle of position b*R1 + m (stage-0 group m,
+            // index b) is applied here: its product leaves the value reduced,
+            // so only the untwiddled positions (m*b == 0) need a reduction
 #pragma unroll
-            for (int m = 0; m < R1; m++) lds[g * LT + T2Lds<LOGT, COLS>::pos(b * R1 + m)] = AR::to_lds(x[m], P);
+            for (int m = 0; m < R1; m++) {
+                const int e = m * b;
+                lds[g * LT + T2Lds<LOGT, COLS>::pos(b * R1 + m)] =
+                    e == 0 ? AR::to_lds(x[m], P) : AR::to_lds_r(AR::mul(x[m], twl[T - e], P));
+            }
         }
     };
-    // stage 0 inverse: LDS -> inverse stage twiddle -> inverse DFT_R0 -> HBM
+    // stage 0 inverse: LDS (stage twiddles already applied) -> inverse DFT_R0 -> HBM
     auto inv_stage0 = [&]() {
         V x[R0];
 #pragma unroll
         for (int k = 0; k < R0; k++) x[brev(k, R0LOG)] = AR::from_lds(lds[g0 * LT + T2Lds<LOGT, COLS>::pos(k * R1 + j0)]);
-#pragma unroll
-        for (int k = 1; k < R0; k++)
-            x[brev(k, R0LOG)] = AR::mul(x[brev(k, R0LOG)], twl[(T - j0 * k) & (T - 1)], P);
-        dft32_dit_inv<AR, R0LOG>(x, pc);
+        dft32_dit_inv<AR, R0LOG>(x, pc);     // stage twiddles were applied before the transpose
         W* out = reinterpret_cast<W*>(prm.out[0]) + plane_off + line_base(g0) + j0 * tstride;
         const int64_t step = (int64_t)R1 * tstride;
         if (canon) {

@@ -88,6 +88,15 @@ const char* bw_last_error(void);
 const char* bw_version(void);
 /* hipSetDevice + table warm-up for the device; optional. */
 int bw_init(int device);
+/* Precompute cache (docs/TIMING.md:24-26,98-100: the host tables are the
+ * disk-cache payload of precompute_ntt).  bw_tables_save writes the host
+ * twiddle tables and constants of a prime set (BW_SET_*) to a file;
+ * bw_tables_load reads one back (reporting its set) so that tables for that
+ * set are uploaded from it, not rebuilt, on every device that has not yet
+ * built them.  Files of another format or library, or damaged ones, are
+ * rejected (BW_ERR_ARG).  Host only: no GPU call. */
+int bw_tables_save(int set, const char* path);
+int bw_tables_load(const char* path, int* set);
 
 /* ---- size policy (src/ntt_limits.cpp) ---------------------------------- */
 uint64_t bw_padded_ntt_size(uint64_t L_A, uint64_t L_B);                         /* :145-152 */

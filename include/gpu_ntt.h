@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <chrono>
+#include <string>
 #include <vector>
 #include <hip/hip_runtime_api.h>
 
@@ -127,6 +128,35 @@ inline NTTPrecomputed precompute_ntt(size_t N, PrecomputeTiming* timing_out = nu
         timing_out->total_ms = (float)(t2 - t0);
     }
     return pre;
+}
+
+// Precompute cache (docs/TIMING.md:24-26,98-100).  The host tables are per
+// prime set, not per N: save_ntt_precomputed writes the images of the
+// precompute's set and of every set the library picks for products to
+// `dir`/bw_tables_<set>.bin; load_ntt_precomputed reads the images found there
+// (in a later process, before upload_ntt_precomputed) and returns the
+// precompute for N.  Both return false / exit on a missing or damaged image.
+inline bool save_ntt_precomputed(const NTTPrecomputed& pre, const std::string& dir) {
+    for (int set : {pre.prime_set, (int)BW_SET_D64}) {
+        const std::string path = dir + "/bw_tables_" + std::to_string(set) + ".bin";
+        if (bw_tables_save(set, path.c_str()) != BW_OK) {
+            std::fprintf(stderr, "[NTT] save_ntt_precomputed: %s\n", bw_last_error());
+            return false;
+        }
+    }
+    return true;
+}
+
+inline NTTPrecomputed load_ntt_precomputed(size_t N, const std::string& dir, PrecomputeTiming* timing_out = nullptr) {
+    for (int set = 0; set < 16; set++) {
+        const std::string path = dir + "/bw_tables_" + std::to_string(set) + ".bin";
+        if (FILE* f = std::fopen(path.c_str(), "rb")) {
+            std::fclose(f);
+            int got = -1;
+            if (bw_tables_load(path.c_str(), &got) != BW_OK || got != set) bw_dropin::die("load_ntt_precomputed");
+        }
+    }
+    return precompute_ntt(N, timing_out);
 }
 
 // Twiddle tables are built and uploaded once per prime set and device (the

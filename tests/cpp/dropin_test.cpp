@@ -18,6 +18,7 @@
 #include <gmp.h>
 
 #include <cstring>
+#include <unistd.h>
 #include <random>
 #include <string>
 
@@ -86,6 +87,23 @@ static void host_checks() {
     const CRTGarnerParams g = compute_garner_params(moduli);
     for (int i = 1; i < NUM_MODULI; i++)
         CHECK((unsigned __int128)g.inv[i] * g.prefix_M[i] % g.primes[i] == 1);
+    // precompute cache: save the host tables, load them back (host only)
+    {
+        char dir[] = "/tmp/bw_dropin_XXXXXX";
+        CHECK(mkdtemp(dir) != nullptr);
+        const NTTPrecomputed pre = precompute_ntt(4096);
+        CHECK(save_ntt_precomputed(pre, dir));
+        PrecomputeTiming t{};
+        const NTTPrecomputed back = load_ntt_precomputed(4096, dir, &t);
+        CHECK(back.prime_set == pre.prime_set && back.primes == pre.primes && back.N == 4096);
+        for (int set : {pre.prime_set, (int)BW_SET_D64}) {
+            const std::string path = std::string(dir) + "/bw_tables_" + std::to_string(set) + ".bin";
+            int got = -1;
+            CHECK(bw_tables_load(path.c_str(), &got) == BW_OK && got == set);
+            std::remove(path.c_str());
+        }
+        rmdir(dir);
+    }
 }
 
 static void hip_ok(hipError_t e) {
