@@ -32,7 +32,7 @@ __all__ = [
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbirdwing_hip.so")
 
-SET_DEFAULT, SET_U32_REF, SET_U32_WIDE, SET_U64_REF, SET_HYBRID, SET_D64 = -1, 0, 1, 2, 3, 4
+SET_DEFAULT, SET_U32_REF, SET_U32_WIDE, SET_U64_REF, SET_HYBRID, SET_D64, SET_U64_PAIRS = -1, 0, 1, 2, 3, 4, 5
 SCHED_PASS_MAJOR, SCHED_PRIME_MAJOR, SCHED_HYBRID = 0, 1, 2
 BW_OK, BW_ERR_ARG, BW_ERR_UNSUPPORTED_SIZE, BW_ERR_HIP, BW_ERR_NOMEM, BW_ERR_STATE = 0, -1, -2, -3, -4, -5
 

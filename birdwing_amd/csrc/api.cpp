@@ -101,7 +101,7 @@ extern "C" int bw_crt_bound_satisfied(int set, int in_bits, uint64_t L_A, uint64
     return ok;
 }
 extern "C" int bw_multiply_size_supported(int set, int in_bits, uint64_t L_A, uint64_t L_B, char* why, size_t len) {
-    if (set < 0) set = default_set(in_bits, padded_ntt_size(L_A, L_B));
+    if (set < 0) set = auto_set(in_bits, L_A ? L_A : 1, L_B ? L_B : 1);   // the set a plan would pick
     const PrimeSet* s = prime_set(set);
     if (!s) return 0;
     std::string w;
@@ -182,8 +182,8 @@ extern "C" int bw_plan_create_ex(bw_plan** out, int limb_bits, uint64_t L_A, uin
     if (set_id < 0) set_id = auto_set(limb_bits, L_A, L_B);
     const PrimeSet* s = prime_set(set_id);
     if (!s) return fail(BW_ERR_ARG, "unknown prime set %d", set_id);
-    const bool d64 = s->digit_bits == 64;
-    if (d64 ? s->k != 5 : (s->limb_bits != limb_bits || s->k != 3))
+    const bool d64 = s->digit_bits == 64;      // 64-bit digits (limb pairs or 64-bit limbs)
+    if (d64 ? !(s->k == 5 || s->id == SET_U64_PAIRS) : (s->limb_bits != limb_bits || s->k != 3))
         return fail(BW_ERR_ARG, "prime set %d does not serve %d-bit limbs", set_id, limb_bits);
     std::string why;
     if (!multiply_size_supported(*s, limb_bits, L_A, L_B, &why))
@@ -355,7 +355,7 @@ extern "C" int bw_execute(bw_plan* p, const void* d_a, const void* d_b, void* d_
     if ((rc = run_plan_transforms(p, d_a, d_b, st, mk))) return rc;
     CarryParams cp;
     std::memset(&cp, 0, sizeof(cp));
-    cp.src = p->ingress == 2 ? BW_SRC_GARNER_D64 : (p->arith_bits == 64 ? BW_SRC_GARNER_U64 : BW_SRC_GARNER_U32);
+    cp.src = p->set->k == 5 ? BW_SRC_GARNER_D64 : (p->arith_bits == 64 ? BW_SRC_GARNER_U64 : BW_SRC_GARNER_U32);
     cp.gd = p->tabs->gd;
     cp.half_last = p->half_last;
     cp.a0 = p->bufA;
@@ -380,8 +380,8 @@ extern "C" int bw_execute_crt(bw_plan* p, const void* d_a, const void* d_b, uint
     int rc = check_plan(p);
     if (rc) return rc;
     if (p->ingress == 2)
-        return fail(BW_ERR_STATE, "plan runs 64-bit digits under five primes; CRT coefficients of the "
-                                  "reference pipelines need a plan with an explicit three-prime set");
+        return fail(BW_ERR_STATE, "plan runs 64-bit digits (limb pairs); CRT coefficients of the "
+                                  "reference pipelines need a plan with an explicit reference prime set");
     if (!d_a || !d_b || !d_c0 || !d_c1 || (p->limb_bits == 64 && !d_c2)) return fail(BW_ERR_ARG, "NULL device buffer");
     hipStream_t st = (hipStream_t)stream;
     Marker mk(p, st);

@@ -270,8 +270,8 @@ int batch_issue(const BatchLayout& L, void* ws, const void* dA, const void* dB, 
         LAUNCH_TRY(bw_launch_carry_reset(flags, c.ntiles, counter, st));
         CarryParams cp;
         std::memset(&cp, 0, sizeof(cp));
-        cp.src = prime_set(c.set_id)->digit_bits == 64 ? BW_SRC_GARNER_D64
-                 : (c.limb_bits == 64 ? BW_SRC_GARNER_U64 : BW_SRC_GARNER_U32);
+        cp.src = prime_set(c.set_id)->k == 5 ? BW_SRC_GARNER_D64
+                 : (prime_set(c.set_id)->limb_bits == 64 ? BW_SRC_GARNER_U64 : BW_SRC_GARNER_U32);
         cp.gd = tabs->gd;
         cp.out = dC;
         cp.g = tabs->garner;

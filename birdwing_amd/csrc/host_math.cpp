@@ -17,6 +17,12 @@ namespace bw {
 // 32-bit digits this halves N for five transforms instead of three (about 0.8
 // of the butterflies and of the residue bytes) and still runs in FP64.  Roots
 // are g^((p-1)/2^v2) for the primitive roots g = 5, 31, 13, 29, 10.
+//
+// The limb-pair set runs the reference's 58-bit primes on 64-bit digits (a
+// pair of 32-bit limbs or one 64-bit limb, coefficients < N * 2^128 < 2^174):
+// the reach of the reference's hybrid pipeline (32-bit limbs under 58-bit
+// primes, include/config.h:26-30, N <= 2^33 limb positions) at three
+// transforms of N/2 64-bit words.
 static const PrimeSet kSets[SET_COUNT] = {
     {SET_U32_REF, 32, 3, {0x2d000001ULL, 0x23800001ULL, 0x26800001ULL}, {663, 721, 19}, 0},
     {SET_U32_WIDE, 32, 3, {2013265921ULL, 1811939329ULL, 469762049ULL}, {440564289ULL, 72705542ULL, 2187ULL}, 0},
@@ -26,6 +32,8 @@ static const PrimeSet kSets[SET_COUNT] = {
      {273765203699653965ULL, 26231613454922890ULL}, 0},
     {SET_D64, 32, 5, {2113929217ULL, 2013265921ULL, 1811939329ULL, 1711276033ULL, 1107296257ULL},
      {1971140334ULL, 440564289ULL, 72705542ULL, 969788637ULL, 1087287097ULL}, 64},
+    {SET_U64_PAIRS, 64, 3, {0x400002600000001ULL, 0x400004200000001ULL, 0x400001100000001ULL},
+     {273765203699653965ULL, 26231613454922890ULL, 144261359744151243ULL}, 64},
 };
 
 const PrimeSet* prime_set(int id) { return (id >= 0 && id < SET_COUNT) ? &kSets[id] : nullptr; }
@@ -45,7 +53,11 @@ uint64_t padded_size_for(const PrimeSet& s, int in_bits, uint64_t L_A, uint64_t 
 
 int auto_set(int limb_bits, uint64_t L_A, uint64_t L_B) {
     if (multiply_size_supported(kSets[SET_D64], limb_bits, L_A, L_B, nullptr)) return SET_D64;
-    return default_set(limb_bits, padded_ntt_size(L_A, L_B));
+    const int d = default_set(limb_bits, padded_ntt_size(L_A, L_B));
+    if (limb_bits == 32 && !multiply_size_supported(kSets[d], limb_bits, L_A, L_B, nullptr) &&
+        multiply_size_supported(kSets[SET_U64_PAIRS], limb_bits, L_A, L_B, nullptr))
+        return SET_U64_PAIRS;
+    return d;
 }
 
 uint64_t mulmod(uint64_t a, uint64_t b, uint64_t p) { return (uint64_t)(((u128)a * b) % p); }

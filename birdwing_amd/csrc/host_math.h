@@ -8,7 +8,8 @@
 
 namespace bw {
 
-enum { SET_U32_REF = 0, SET_U32_WIDE = 1, SET_U64_REF = 2, SET_HYBRID = 3, SET_D64 = 4, SET_COUNT = 5 };
+enum { SET_U32_REF = 0, SET_U32_WIDE = 1, SET_U64_REF = 2, SET_HYBRID = 3, SET_D64 = 4, SET_U64_PAIRS = 5,
+       SET_COUNT = 6 };
 
 struct PrimeSet {
     int id;
@@ -21,8 +22,9 @@ struct PrimeSet {
 
 const PrimeSet* prime_set(int id);
 int default_set(int limb_bits, uint64_t N);
-// Prime set a product without a requested set runs under: the 64-bit-digit
-// set when it supports the sizes, else default_set().
+// Prime set a product without a requested set runs under: the five-prime
+// 64-bit-digit set when it supports the sizes, else default_set() when that
+// does, else (32-bit limbs beyond both) the 58-bit primes on limb pairs.
 int auto_set(int limb_bits, uint64_t L_A, uint64_t L_B);
 // Digits of an operand of L limbs of in_bits under set s.
 uint64_t digits_of(const PrimeSet& s, int in_bits, uint64_t L);

@@ -120,6 +120,8 @@ struct ArU64x {
     __device__ static V from_lds(LV x) { return x; }
     __device__ static V from_word(W x, const PR&) { return x; }
     __device__ static V from_raw(Raw x, const PR& P) { return u64_reduce(x, P); }
+    // a 64-bit digit (a pair of 32-bit limbs or one 64-bit limb)
+    __device__ static V from_raw64(uint64_t d, const PC32& c) { return u64_reduce(d, c.P); }
     __device__ static W to_word(V x, const PR&) { return x; }
     __device__ static W to_word_c(V x, const PR&, bool) { return x; }
     __device__ static LV to_lds_r(V x) { return x; }
@@ -581,22 +583,15 @@ typedef void (*PassKernel2)(PassParams);
 
 template <class AR, int MODE, int LOGT>
 static PassKernel2 pick2_v(bool cols, int ingress) {
-    constexpr bool D64 = std::is_same<AR, ArF64x>::value;   // 64-bit digits: FP64 arithmetic only
     if constexpr (MODE == BW_MODE_INV) {
         if (ingress) return nullptr;
         return cols ? ntt2_kernel<AR, LOGT, MODE, true, 0> : ntt2_kernel<AR, LOGT, MODE, false, 0>;
     } else if constexpr (MODE == BW_MODE_FUSED) {
         if (cols) return nullptr;
-        if (ingress == 2) {
-            if constexpr (D64) return ntt2_kernel<AR, LOGT, MODE, false, 2>;
-            else return nullptr;
-        }
+        if (ingress == 2) return ntt2_kernel<AR, LOGT, MODE, false, 2>;
         return ingress ? ntt2_kernel<AR, LOGT, MODE, false, 1> : ntt2_kernel<AR, LOGT, MODE, false, 0>;
     } else {
-        if (ingress == 2) {
-            if constexpr (D64) return cols ? ntt2_kernel<AR, LOGT, MODE, true, 2> : ntt2_kernel<AR, LOGT, MODE, false, 2>;
-            else return nullptr;
-        }
+        if (ingress == 2) return cols ? ntt2_kernel<AR, LOGT, MODE, true, 2> : ntt2_kernel<AR, LOGT, MODE, false, 2>;
         if (cols) return ingress ? ntt2_kernel<AR, LOGT, MODE, true, 1> : ntt2_kernel<AR, LOGT, MODE, true, 0>;
         return ingress ? ntt2_kernel<AR, LOGT, MODE, false, 1> : ntt2_kernel<AR, LOGT, MODE, false, 0>;
     }

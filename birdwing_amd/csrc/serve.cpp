@@ -343,7 +343,7 @@ extern "C" int bw_server_create(bw_server** out, const bw_server_config* cfg) {
         if (e != hipSuccess) rc = fail(BW_ERR_HIP, "server streams: %s", hipGetErrorString(e));
     }
     // tables of every prime set a request can need, before the first request
-    for (int set : {SET_U32_REF, SET_U32_WIDE, SET_U64_REF, SET_D64}) {
+    for (int set : {SET_U32_REF, SET_U32_WIDE, SET_U64_REF, SET_D64, SET_U64_PAIRS}) {
         DevTables* t;
         if (!rc) rc = get_tables(set, &t);
     }

@@ -13,7 +13,7 @@
  * reference, include/config.h:17-53):
  *   limb_bits = 32: 32-bit limbs, three ~30-bit primes, 128-bit Garner.
  *   limb_bits = 64: 64-bit limbs, three 58-bit primes, 160-bit Garner.
- * Prime sets (src/gpu_ntt.cu:28-47 plus two of this library's):
+ * Prime sets (src/gpu_ntt.cu:28-47 plus three of this library's):
  *   BW_SET_U32_REF  {0x2d000001, 0x23800001, 0x26800001}, N <= 2^23
  *   BW_SET_U32_WIDE {2013265921, 1811939329, 469762049},  N <= 2^26
  *   BW_SET_U64_REF  {0x400002600000001, 0x400004200000001, 0x400001100000001}
@@ -22,14 +22,19 @@
  *                   64-bit digits (a pair of 32-bit limbs or one 64-bit limb),
  *                   five transforms of half the length, FP64 arithmetic,
  *                   digit transforms of up to 2^25 points.
+ *   BW_SET_U64_PAIRS the BW_SET_U64_REF primes on 64-bit digits (pairs of
+ *                   32-bit limbs): the reach of the reference's hybrid
+ *                   pipeline for 32-bit limbs (include/config.h:26-30), up to
+ *                   2^32 limbs per operand (memory permitting).
  * bw_default_prime_set() is the three-prime set of the reference pipeline for
  * a 32-bit-digit length N: BW_SET_U32_REF while N <= 2^23, BW_SET_U32_WIDE
  * above (u32), BW_SET_U64_REF (u64).  A plan created with prime_set = -1 runs
  * under BW_SET_D64 when that set supports the sizes (fewer butterflies and
- * residue bytes for the same product) and under the default set otherwise;
- * every product is exact either way, but only three-prime plans give the
- * reference's CRT coefficients (bw_execute_crt).  Arrays receiving primes
- * hold at least 5 entries.
+ * residue bytes for the same product), else under the default set when that
+ * does, else (32-bit limbs) under BW_SET_U64_PAIRS; every product is exact
+ * either way, but only the reference's sets give its CRT coefficients
+ * (bw_execute_crt).  bw_multiply_size_supported(-1, ...) answers for that
+ * automatic choice.  Arrays receiving primes hold at least 5 entries.
  *
  * Streams are hipStream_t values passed as void* (NULL = default stream).
  * Device entry points are asynchronous on that stream; *_host entry points
@@ -59,7 +64,7 @@ enum {
 enum { BW_SCHED_PASS_MAJOR = 0, BW_SCHED_PRIME_MAJOR = 1, BW_SCHED_HYBRID = 2 };
 
 enum { BW_SET_DEFAULT = -1, BW_SET_U32_REF = 0, BW_SET_U32_WIDE = 1, BW_SET_U64_REF = 2, BW_SET_HYBRID = 3,
-       BW_SET_D64 = 4 };
+       BW_SET_D64 = 4, BW_SET_U64_PAIRS = 5 };
 
 typedef struct bw_plan bw_plan;
 

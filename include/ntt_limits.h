@@ -2,9 +2,11 @@
 // (src/ntt_limits.cpp), answered by libbirdwing_hip.so's size policy.
 //
 // The configured limits are this library's: for 32-bit limbs the reference
-// prime set serves N <= 2^23 and the wide set {2013265921, 1811939329,
-// 469762049} extends it to N = 2^26 (so max_supported_logN() == 26 where the
-// reference reports 23); for 64-bit limbs the reference 58-bit set (N <= 2^32).
+// prime set serves N <= 2^23, the wide set {2013265921, 1811939329, 469762049}
+// N <= 2^26, and the 58-bit primes on limb pairs N <= 2^33 limb positions (so
+// max_supported_logN() == 33, the reach of the reference's hybrid pipeline,
+// where its 32-bit pipeline reports 23); for 64-bit limbs the reference 58-bit
+// set (N <= 2^32).
 // The per-prime helpers are exact restatements.  ensure_*() print to stderr
 // and exit(1), as in the reference.
 #pragma once
@@ -18,8 +20,11 @@
 #include "config.h"
 
 namespace bw_dropin {
-inline int widest_set() { return BW_CFG_LIMB_BITS == 64 ? BW_SET_U64_REF : BW_SET_U32_WIDE; }
-inline int set_for(size_t N) { return bw_default_prime_set(BW_CFG_LIMB_BITS, N); }
+// 32-bit limbs: lengths beyond the 32-bit-digit sets run as limb pairs
+// (half as many 64-bit digits) under BW_SET_U64_PAIRS
+inline int widest_set() { return BW_CFG_LIMB_BITS == 64 ? BW_SET_U64_REF : BW_SET_U64_PAIRS; }
+inline bool paired(size_t N) { return BW_CFG_LIMB_BITS == 32 && N > ((size_t)1 << 26); }
+inline int set_for(size_t N) { return paired(N) ? BW_SET_U64_PAIRS : bw_default_prime_set(BW_CFG_LIMB_BITS, N); }
 }  // namespace bw_dropin
 
 inline int max_logN_for_prime(uint64_t p) {
@@ -39,7 +44,9 @@ inline int max_root_logN(TestDataTypeUint root, TestDataTypeUint p) {
     return 62;
 }
 
-inline int max_supported_logN() { return bw_max_supported_logN(bw_dropin::widest_set()); }
+inline int max_supported_logN() {
+    return bw_max_supported_logN(bw_dropin::widest_set()) + (BW_CFG_LIMB_BITS == 32 ? 1 : 0);   // limb positions
+}
 inline size_t max_supported_N() { return size_t(1) << max_supported_logN(); }
 inline size_t max_supported_limb_count() {
     return (size_t)bw_max_supported_limb_count(bw_dropin::widest_set(), INPUT_LIMB_BITS);
@@ -48,8 +55,9 @@ inline size_t padded_ntt_size(size_t L_A, size_t L_B) { return (size_t)bw_padded
 
 inline bool ntt_size_supported(size_t N, std::string* why = nullptr) {
     char buf[256] = {0};
-    const int ok = (N && !(N & (N - 1))) ? bw_ntt_size_supported(bw_dropin::set_for(N), N, buf, sizeof buf)
-                                         : bw_ntt_size_supported(BW_SET_U32_REF, N, buf, sizeof buf);
+    const int ok = (N && !(N & (N - 1)))
+                       ? bw_ntt_size_supported(bw_dropin::set_for(N), bw_dropin::paired(N) ? N / 2 : N, buf, sizeof buf)
+                       : bw_ntt_size_supported(BW_SET_U32_REF, N, buf, sizeof buf);
     if (!ok && why) *why = buf;
     return ok != 0;
 }

@@ -124,6 +124,27 @@ def test_d64_prime_set():
     assert bw.max_supported_limb_count(bw.SET_D64, 32) == 1 << 25
 
 
+def test_limb_pair_prime_set():
+    """32-bit limbs beyond the 32-bit-digit sets run as limb pairs under the
+    reference's 58-bit primes (the reach of its hybrid pipeline,
+    include/config.h:26-30): coefficients < N * 2^128 below their product,
+    transforms up to 2^32 digits (2-adic valuation of the third prime)."""
+    import birdwing_amd as bw
+    primes, roots = bw.prime_set_info(bw.SET_U64_PAIRS)
+    assert (primes, roots) == bw.prime_set_info(bw.SET_U64_REF)
+    prod = primes[0] * primes[1] * primes[2]
+    assert prod > (1 << 32) * (2**64 - 1) ** 2
+    assert bw.max_supported_logN(bw.SET_U64_PAIRS) == 32
+    assert bw.max_supported_limb_count(bw.SET_U64_PAIRS, 32) == 1 << 32
+    assert bw.multiply_size_supported(bw.SET_U64_PAIRS, 32, 1 << 32, 1 << 32)[0]
+    assert not bw.multiply_size_supported(bw.SET_U64_PAIRS, 32, (1 << 32) + 2, (1 << 32) + 2)[0]
+    # the automatic choice (-1): the five-prime set through 2^25 limbs per
+    # operand, limb pairs beyond it, nothing past 2^32
+    for L, ok in ((1 << 25, True), ((1 << 25) + 1, True), (1 << 30, True), ((1 << 32) + 2, False)):
+        assert bw.multiply_size_supported(-1, 32, L, L)[0] == ok, L
+    assert not bw.multiply_size_supported(bw.SET_U32_WIDE, 32, (1 << 25) + 1, (1 << 25) + 1)[0]
+
+
 def test_abi_version_matches_header():
     import re
     import birdwing_amd as bw

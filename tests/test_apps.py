@@ -73,10 +73,11 @@ def test_bench_rejects_bad_spec_and_skips_unsupported():
     for bad in ("8-6", "x", "0", "5-"):
         r = _run([exe, "--list", bad])
         assert r.returncode == 1 and "Invalid limb spec" in r.stderr, bad
-    r = _run([exe, "--list", "10", "31"])      # 2^31 limbs: beyond every 32-bit-limb set
+    # 2^31 limbs run as limb pairs under the 58-bit primes; 2^33 is beyond every set
+    r = _run([exe, "--list", "10", "31", "33"])
     assert r.returncode == 0
-    assert [t[0] for t in _listed(r.stdout)] == [10]
-    assert "Skipping L_arg=31" in r.stderr
+    assert [t[0] for t in _listed(r.stdout)] == [10, 31]
+    assert "Skipping L_arg=33" in r.stderr
     r = _run([exe, "--iters", "0", "--list"])
     assert r.returncode == 1
 

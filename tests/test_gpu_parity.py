@@ -381,8 +381,13 @@ def test_host_entry_points(gpu, oracle):
 
 
 def test_unsupported_sizes_raise(gpu):
+    # beyond every set (limb pairs reach 2^32 limbs per operand); rejected
+    # before any allocation
     with pytest.raises(gpu.BWError) as e:
-        gpu.Plan(32, 1 << 26, 1 << 26)
+        gpu.Plan(32, (1 << 32) + 2, (1 << 32) + 2)
+    assert e.value.code == gpu.BW_ERR_UNSUPPORTED_SIZE
+    with pytest.raises(gpu.BWError) as e:
+        gpu.Plan(32, 1 << 26, 1 << 26, gpu.SET_U32_WIDE)
     assert e.value.code == gpu.BW_ERR_UNSUPPORTED_SIZE
     with pytest.raises(gpu.BWError):
         gpu.Plan(32, 0, 5)
@@ -509,3 +514,67 @@ def test_three_prime_products_vs_gmp(gpu, oracle, bits, LA, LB):
     s = gpu.default_prime_set(bits, gpu.padded_ntt_size(LA, LB))
     plan = _product_check(gpu, oracle, A, B, bits, s)
     assert plan.k == 3
+
+
+# ---- 32-bit limbs as limb pairs under the 58-bit primes ----------------------
+@pytest.mark.parametrize("bits,LA,LB", [(32, 1, 1), (32, 3, 2), (32, 1000, 999), (32, 4097, 3000),
+                                        (32, 1 << 16, (1 << 16) - 1), (32, 1 << 20, 1 << 20),
+                                        (64, 2000, 1999), (64, 1 << 17, 5)])
+def test_limb_pairs_vs_gmp(gpu, oracle, bits, LA, LB):
+    """BW_SET_U64_PAIRS: 64-bit digits (odd 32-bit limb counts leave a half
+    digit) in 64-bit integer arithmetic, against GMP."""
+    A = oracle.uniform_limbs(LA, 500 + LA, bits)
+    B = oracle.uniform_limbs(LB, 501 + LB, bits)
+    plan = _product_check(gpu, oracle, A, B, bits, gpu.SET_U64_PAIRS)
+    assert plan.info.prime_set == gpu.SET_U64_PAIRS and plan.info.digit_bits == 64
+
+
+def _value_mod(limbs, q, block=1 << 14):
+    """A little-endian 32-bit limb array's value mod q (q < 2^31), vectorized
+    over blocks: sum_i a_i 2^(32 i) = sum_h 2^(32 B h) sum_l a_(hB+l) 2^(32 l)."""
+    a = np.asarray(limbs, dtype=np.uint32)
+    n = -(-len(a) // block) * block
+    x = np.zeros(n, dtype=np.uint64)
+    x[:len(a)] = a
+    x %= np.uint64(q)
+    base = pow(2, 32, q)
+    pw = np.empty(block, dtype=np.uint64)
+    acc = 1
+    for i in range(block):
+        pw[i] = acc
+        acc = acc * base % q
+    inner = ((x.reshape(-1, block) * pw) % np.uint64(q)).sum(axis=1) % np.uint64(q)
+    step = pow(2, 32 * block, q)
+    total, w = 0, 1
+    for v in inner.tolist():
+        total = (total + v * w) % q
+        w = w * step % q
+    return total
+
+
+def test_limb_pairs_auto_beyond_five_primes(gpu, oracle):
+    """2^25 + 5 limbs per operand: past the five-prime set (2^25 limbs) and the
+    32-bit-digit sets (N <= 2^26), so the automatic choice is the limb-pair set.
+    Checked by residues mod three primes and by the exact low 2048 limbs."""
+    L = (1 << 25) + 5
+    rng = np.random.default_rng(77)
+    A = rng.integers(0, 2 ** 32, L, dtype=np.uint64).astype(np.uint32)
+    B = rng.integers(0, 2 ** 32, L - 3, dtype=np.uint64).astype(np.uint32)
+    A[-1] |= 1
+    B[-1] |= 1
+    d = dev()
+    plan = gpu.Plan(32, len(A), len(B))
+    assert plan.info.prime_set == gpu.SET_U64_PAIRS
+    print("limb pairs: %d x %d limbs, N = 2^%d digits" % (len(A), len(B), plan.info.N_exec.bit_length() - 1),
+          flush=True)
+    c = d.empty(len(A) + len(B), 32)
+    plan.execute(d.to_dev(A), d.to_dev(B), c)
+    got = d.from_dev(c, len(A) + len(B))
+    plan.status()
+    plan.close()
+    for q in (2147483647, 2147483629, 2147483587):
+        assert _value_mod(got, q) == _value_mod(A, q) * _value_mod(B, q) % q, q
+    m = 2048
+    low = lambda v: int.from_bytes(np.ascontiguousarray(v[:m]).tobytes(), "little")
+    want_low = (low(A) * low(B)) % (1 << (32 * m))
+    assert low(got) == want_low
