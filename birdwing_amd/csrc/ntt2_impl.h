@@ -318,10 +318,11 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         else return AR::from_raw(w, P);
     };
     // This thread's R0 stage-0 input words of one operand: digits base + m*step.
-    // Raw operands are zero past their end.  Three cases: every digit inside
-    // (loads without per-element tests, 64-bit when the limb pair is 8-byte
-    // aligned), every digit past the end (no loads), and digits straddling the
-    // end (per-element tests).
+    // Raw operands are zero past their end.  Cases: every digit inside (loads
+    // without per-element tests, 64-bit when the limb pair is 8-byte aligned),
+    // every digit past the end (no loads), the lower half inside and the upper
+    // half past the end (a zero-padded operand), and digits straddling the end
+    // elsewhere (per-element tests).
     auto load0 = [&](int which, IW* w) -> bool {
         int64_t base = line_base(g0) + j0 * tstride;
         const int64_t step = (int64_t)R1 * tstride;
@@ -352,6 +353,26 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
             } else if (LW * base >= len) {
 #pragma unroll
                 for (int m = 0; m < R0; m++) w[m] = (IW)0;
+            } else if (upper_zero && LW * (base + (int64_t)(R0 / 2 - 1) * step) + (LW - 1) < len) {
+                // the common zero-padded case: lower half inside, upper half past the end
+                if constexpr (INGRESS == 2) {
+                    if ((reinterpret_cast<uintptr_t>(raw) & 7) == 0) {
+                        const uint64_t* r64 = reinterpret_cast<const uint64_t*>(raw);
+#pragma unroll
+                        for (int m = 0; m < R0 / 2; m++) w[m] = r64[base + m * step];
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < R0 / 2; m++) {
+                            const int64_t a = 2 * (base + m * step);
+                            w[m] = (uint64_t)raw[a] | ((uint64_t)raw[a + 1] << 32);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int m = 0; m < R0 / 2; m++) w[m] = (IW)raw[base + m * step];
+                }
+#pragma unroll
+                for (int m = R0 / 2; m < R0; m++) w[m] = (IW)0;
             } else {
                 // straddles the end: digits below dfull are whole, digit dfull
                 // may hold only its low limb (odd limb count)
@@ -500,14 +521,14 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
             }
             dft32_dit_inv<AR, R1LOG>(x, pc);
             // the inverse stage twiddle of position b*R1 + m (stage-0 group m,
-            // index b) is applied here: its product leaves the value reduced,
-            // so only the untwiddled positions (m*b == 0) need a reduction
+            // index b) is applied here: its product leaves the value reduced.
+            // m = 0 is untwiddled (a reduction); b = 0 multiplies by twl[0] = 1,
+            // which keeps the loop free of branches.
+            lds[g * LT + T2Lds<LOGT, COLS>::pos(b * R1)] = AR::to_lds(x[0], P);
 #pragma unroll
-            for (int m = 0; m < R1; m++) {
-                const int e = m * b;
+            for (int m = 1; m < R1; m++)
                 lds[g * LT + T2Lds<LOGT, COLS>::pos(b * R1 + m)] =
-                    e == 0 ? AR::to_lds(x[m], P) : AR::to_lds_r(AR::mul(x[m], twl[T - e], P));
-            }
+                    AR::to_lds_r(AR::mul(x[m], twl[(T - m * b) & (T - 1)], P));
         }
     };
     // stage 0 inverse: LDS (stage twiddles already applied) -> inverse DFT_R0 -> HBM
