@@ -500,7 +500,7 @@ def main():
     stream = torch.cuda.current_stream()
     tuned = False
     if not passes and not args.no_tune:
-        plan.tune(da, db, stream, reps=2)      # pass split / tile cap / launch order, before any timing
+        plan.tune(da, db, stream, reps=3)      # pass split / tile cap / launch order, before any timing
         tuned = True
     if args.schedule >= 0:
         plan.set_schedule(args.schedule)

@@ -460,7 +460,10 @@ extern "C" int bw_plan_tune(bw_plan* p, const void* d_a, const void* d_b, void* 
             (void)hipEventElapsedTime(&ms, e0, e1);
             if (r > 0 && (ms_min < 0 || ms < ms_min)) ms_min = ms;    // first run warms up
         }
-        if (best.ms < 0 || ms_min < best.ms) {
+        // the incumbent (the plan's current choice, timed first) keeps its place
+        // unless a candidate is clearly faster: run-to-run noise of a few
+        // percent must not flip the split or the schedule
+        if (best.ms < 0 || ms_min < best.ms * 0.98f) {
             best = c;
             best.ms = ms_min;
         }

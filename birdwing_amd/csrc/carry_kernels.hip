@@ -91,28 +91,32 @@ struct SrcGarnerU32 {
     using Word = uint32_t;
     static constexpr int NW = 3;
     __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
-        if (n < 0 || n >= p.N) { w[0] = w[1] = w[2] = 0; return; }
+        const bool in = n >= 0 && n < p.N;      // no branch: loads of later rows can issue early
+        const int64_t m = in ? n : 0;
         const uint32_t* r = reinterpret_cast<const uint32_t*>(p.a0);
         uint64_t lo; uint32_t hi;
-        garner_u32(r[n], r[p.plane + n], r[2 * p.plane + n], c.g, c.fp, lo, hi);
-        w[0] = (uint32_t)lo; w[1] = (uint32_t)(lo >> 32); w[2] = hi;
+        garner_u32(r[m], r[p.plane + m], r[2 * p.plane + m], c.g, c.fp, lo, hi);
+        w[0] = in ? (uint32_t)lo : 0u; w[1] = in ? (uint32_t)(lo >> 32) : 0u; w[2] = in ? hi : 0u;
     }
 };
 struct SrcGarnerU64 {
     using Word = uint64_t;
     static constexpr int NW = 3;
     __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
-        if (n < 0 || n >= p.N) { w[0] = w[1] = w[2] = 0; return; }
+        const bool in = n >= 0 && n < p.N;
+        const int64_t m = in ? n : 0;
         const uint64_t* r = reinterpret_cast<const uint64_t*>(p.a0);
-        garner_u64(r[n], r[p.plane + n], r[2 * p.plane + n], c.g, w[0], w[1], w[2]);
+        garner_u64(r[m], r[p.plane + m], r[2 * p.plane + m], c.g, w[0], w[1], w[2]);
+        if (!in) w[0] = w[1] = w[2] = 0;
     }
 };
 struct SrcGarnerD64 {
     using Word = uint64_t;
     static constexpr int NW = 3;
     __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
-        if (n < 0 || n >= p.N) { w[0] = w[1] = w[2] = 0; return; }
-        garner_d64(reinterpret_cast<const uint32_t*>(p.a0), p.plane, n, c.gd, w);
+        const bool in = n >= 0 && n < p.N;
+        garner_d64(reinterpret_cast<const uint32_t*>(p.a0), p.plane, in ? n : 0, c.gd, w);
+        if (!in) w[0] = w[1] = w[2] = 0;
     }
 };
 struct SrcHiLo128 {     // a0 = hi, a1 = lo
@@ -158,7 +162,7 @@ __device__ __forceinline__ Wd shift_up(Wd cur, Wd prev, int j, int lane) {
 #define FLAG_INC 2ULL
 
 template <class SRC>
-__global__ __launch_bounds__(256) void garner_carry_kernel(CarryParams prm) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void garner_carry_kernel(CarryParams prm) {
     using Wd = typename SRC::Word;
     constexpr int NW = SRC::NW;
     constexpr int K = BW_CARRY_ROWS;
