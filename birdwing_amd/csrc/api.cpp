@@ -211,17 +211,17 @@ extern "C" int bw_plan_create_ex(bw_plan** out, int limb_bits, uint64_t L_A, uin
     if (pass_log && npass > 0) {
         int sum = 0;
         for (int i = 0; i < npass; i++) {
-            const int maxlog = d64 ? BW_REG2_MAX_LOG : BW_MAX_PASS_LOG;   // 64-bit digits: register-staged passes
-            if (pass_log[i] < 1 || pass_log[i] > maxlog) {
-                delete p;
-                return fail(BW_ERR_ARG, "pass_log[%d]=%d outside [1, %d]", i, pass_log[i], maxlog);
-            }
             sum += pass_log[i];
             p->passes.push_back(pass_log[i]);
         }
-        if (sum != logN || npass > 4) {
+        std::string why2;
+        if (!split_supported(p->passes, p->arith_bits, p->ingress, &why2)) {
             delete p;
-            return fail(BW_ERR_ARG, "pass split does not add up to log2 N = %d (or more than 4 passes)", logN);
+            return fail(BW_ERR_ARG, "pass split: %s", why2.c_str());
+        }
+        if (sum != logN) {
+            delete p;
+            return fail(BW_ERR_ARG, "pass split does not add up to log2 N = %d", logN);
         }
     } else {
         TuneChoice c;
@@ -441,10 +441,13 @@ extern "C" int bw_plan_tune(bw_plan* p, const void* d_a, const void* d_b, void* 
     TuneChoice best;
     best.ms = -1.0f;
     auto consider = [&](const TuneChoice& c) -> int {
-        bool reg2 = true;
-        for (int t : c.passes) reg2 = reg2 && t <= BW_REG2_MAX_LOG;
-        if (!reg2 && p->ingress == 2) return BW_OK;
-        if (!reg2 && c.passes != p->passes) return BW_OK;
+        // register-staged passes, or a row-resident last pass on the FP64 path
+        bool fast = true;
+        for (size_t i = 0; i < c.passes.size(); i++)
+            fast = fast && (c.passes[i] <= BW_REG2_MAX_LOG ||
+                            (i + 1 == c.passes.size() && c.passes[i] >= BW_ROW_MIN_LOG));
+        if (!split_supported(c.passes, p->arith_bits, p->ingress, nullptr)) return BW_OK;
+        if (!fast && c.passes != p->passes) return BW_OK;
         Xform x = plan_xform(p);
         x.passes = c.passes;
         x.gshift = c.gshift;

@@ -478,6 +478,10 @@ void tune_candidates(int logN, std::vector<TuneChoice>& out) {
         const int r = logN - a;
         if (r >= 2 && r <= 10 && std::abs(a - r) <= 4) add({a, r});
     }
+    // a row-resident last pass of 2^14 or 2^15 points (FP64 path; filtered by
+    // split_supported for the others)
+    for (int t = BW_ROW_MIN_LOG; t <= BW_ROW_MAX_LOG; t++)
+        if (logN - t >= 2 && logN - t <= 10) add({logN - t, t});
     if (logN >= 15) {
         for (int a = 4; a <= 10; a++)
             for (int c2 = 4; c2 <= 10; c2++) {
@@ -506,6 +510,7 @@ extern "C" int bw_batch_tune(bw_batch* b, const void* d_a, const void* d_b, void
         best.ms = -1.0f;
         const int64_t m = (int64_t)g.members.size();
         for (auto& c : cands) {
+            if (!split_supported(c.passes, g.limb_bits, g.ingress, nullptr)) continue;
             BatchGroup t = g;
             t.passes = c.passes;
             t.gshift = c.gshift;

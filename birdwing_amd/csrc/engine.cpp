@@ -157,6 +157,24 @@ hipError_t launch_pass(const Xform& x, int mode, const PassPlan& g, const PassPa
     return bw_launch_pass(L, prm, st);
 }
 
+bool split_supported(const std::vector<int>& passes, int arith_bits, int ingress, std::string* why) {
+    const int P = (int)passes.size();
+    if (P < 1 || P > 4) {
+        if (why) *why = "1 to 4 passes";
+        return false;
+    }
+    const int maxlog = ingress == 2 ? BW_REG2_MAX_LOG : BW_MAX_PASS_LOG;
+    for (int i = 0; i < P; i++) {
+        const int t = passes[i];
+        const bool row = i == P - 1 && P >= 2 && arith_bits == 32 && t >= BW_ROW_MIN_LOG && t <= BW_ROW_MAX_LOG;
+        if (t < 1 || (t > maxlog && !row)) {
+            if (why) *why = "pass " + std::to_string(i) + " of 2^" + std::to_string(t) + " points is outside its kernels";
+            return false;
+        }
+    }
+    return true;
+}
+
 int count_transform_launches(const Xform& x) {
     const int P = (int)x.passes.size();
     if (P == 1) return 1;

@@ -24,6 +24,12 @@ struct DevTables {
 };
 // Tables of prime set `set_id` on the current device (built on first use).
 int get_tables(int set_id, DevTables** out);
+// Whether a pass split can run: every pass within its kernel family (the
+// register-staged kernels up to 2^10 points -- the only ones reading 64-bit
+// digits --, the LDS kernels up to 2^13), and the last pass of a split of two
+// or more may be a row-resident fused pass of 2^14 or 2^15 points on the FP64
+// path.  arith_bits: 32 (FP64 kernels) or 64; ingress as in Xform.
+bool split_supported(const std::vector<int>& passes, int arith_bits, int ingress, std::string* why);
 // Host images of a prime set's tables saved to / loaded from a file (host only).
 int tables_save(int set_id, const char* path);
 int tables_load(const char* path, int* set_id);

@@ -79,6 +79,16 @@ __host__ __device__ constexpr int t2_lds_line(int logT, bool cols) {
                : (1 << logT) + ((1 << logT) >> t2_r1log(logT, cols)) + 1;
 }
 int bw_reg2_lds_line(int logT, bool cols);
+// Row-resident fused pass (row_fused.hip): one row of 2^logT points per
+// workgroup (logT = 14, 15; FP64 arithmetic, 32-bit words).  LDS: the padded
+// line of int32 residues (one pad slot per 16), then 256 twiddle pairs.
+#define BW_ROW_MIN_LOG 14
+#define BW_ROW_MAX_LOG 15
+__host__ __device__ constexpr int row_fused_line_bytes(int logT) {
+    return (((1 << logT) + ((1 << logT) >> 4)) * 4 + 15) / 16 * 16;
+}
+__host__ __device__ constexpr int row_fused_lds_bytes(int logT) { return row_fused_line_bytes(logT) + 256 * 16; }
+hipError_t bw_launch_row_fused(int logT, int64_t rows, int nprimes, const PassParams& prm, hipStream_t stream);
 // LDS of a register-staged workgroup: the tile (G lines of LT values of vbytes,
 // rounded up to 16 bytes), then the T stage twiddles (16-byte pairs).
 __host__ __device__ constexpr int t2_tile_bytes(int G, int lt, int vbytes) { return (G * lt * vbytes + 15) / 16 * 16; }

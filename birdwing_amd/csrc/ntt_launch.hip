@@ -36,6 +36,10 @@ int bw_pass_max_G(int logT, int limb_bits, bool cols) {
 int bw_reg2_lds_line(int logT, bool cols) { return t2_lds_line(logT, cols); }
 
 hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_t stream) {
+    if (L.mode == BW_MODE_FUSED && L.limb_bits == 32 && L.logT >= BW_ROW_MIN_LOG) {
+        if (prm.ingress || prm.bdesc || L.logT > BW_ROW_MAX_LOG) return hipErrorInvalidValue;
+        return bw_launch_row_fused(L.logT, L.tiles, L.nprimes, prm, stream);
+    }
     PassKernel k = nullptr;
     const bool reg2 = L.logT <= BW_REG2_MAX_LOG;   // register-staged kernels
     if (reg2) {

@@ -578,3 +578,48 @@ def test_limb_pairs_auto_beyond_five_primes(gpu, oracle):
     low = lambda v: int.from_bytes(np.ascontiguousarray(v[:m]).tobytes(), "little")
     want_low = (low(A) * low(B)) % (1 << (32 * m))
     assert low(got) == want_low
+
+
+# ---- row-resident fused pass (2^14 / 2^15-point rows) -----------------------
+@pytest.mark.parametrize("bits,LA,LB,pset,passes", [
+    (32, 1 << 15, (1 << 15) - 3, -1, [1, 14]),        # five primes, N = 2^15 digits
+    (32, 1 << 16, 1 << 16, -1, [1, 15]),              # N = 2^16
+    (32, 70000, 60001, -1, [2, 14]),                  # N = 2^16
+    (64, 1 << 16, 1 << 15, -1, [2, 15]),              # 64-bit limbs, N = 2^17
+    (32, 1 << 19, 1 << 19, 1, [6, 14]),               # three primes (wide set), 32-bit digits, N = 2^20
+    (32, 1 << 19, (1 << 19) - 7, 1, [5, 15]),
+    (32, 1 << 21, 1 << 21, -1, [6, 15]),              # N = 2^21
+])
+def test_row_fused_pass_vs_gmp(gpu, oracle, bits, LA, LB, pset, passes):
+    """The last forward pass, the pointwise product and the first inverse pass
+    on whole rows resident in LDS, against GMP."""
+    A = oracle.uniform_limbs(LA, 900 + LA % 97, bits)
+    B = oracle.uniform_limbs(LB, 901 + LB % 89, bits)
+    plan = _product_check(gpu, oracle, A, B, bits, pset, passes)
+    assert plan.passes == passes
+
+
+def test_row_fused_full_size(gpu, oracle):
+    """2^24 x 2^24 limbs with the [9, 15] and [10, 14] splits (three launches)."""
+    A = oracle.uniform_limbs(1 << 24, 41, 32)
+    B = oracle.uniform_limbs(1 << 24, 42, 32)
+    want, _ = oracle.gmp_mul(A, B, 32)
+    d = dev()
+    da, db = d.to_dev(A), d.to_dev(B)
+    for passes in ([9, 15], [10, 14]):
+        print("row split", passes, flush=True)
+        plan = gpu.Plan(32, len(A), len(B), gpu.SET_DEFAULT, passes)
+        c = d.empty(len(A) + len(B), 32)
+        plan.execute(da, db, c)
+        assert np.array_equal(d.from_dev(c, len(A) + len(B)), want), passes
+        plan.close()
+
+
+def test_row_fused_split_rules(gpu):
+    """Row passes are last passes of the FP64 path only."""
+    with pytest.raises(gpu.BWError):
+        gpu.Plan(32, 1 << 14, 1 << 14, gpu.SET_DEFAULT, [15])          # a single pass
+    with pytest.raises(gpu.BWError):
+        gpu.Plan(32, 1 << 20, 1 << 20, gpu.SET_DEFAULT, [15, 5])       # not the last pass
+    with pytest.raises(gpu.BWError):
+        gpu.Plan(64, 1 << 16, 1 << 16, gpu.SET_U64_REF, [3, 14])       # 64-bit arithmetic
