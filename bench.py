@@ -180,12 +180,14 @@ def product_model_bytes(LA, LB, bits, k=3):
     return (int(LA) + int(LB)) * 2 * w + 8 * k * N * w + 2 * N * wc
 
 
-def bench_batch(n, steps, warmup, seed=4096, host_sample=128):
+def bench_batch(n, steps, warmup, seed=4096, host_sample=128, tune=True, compare=True):
     """Batched workload: n products, limb counts per operand log-uniform in
     [2^8, 2^16] (seeded), uniform random 32-bit limbs (top limb nonzero);
     autotuned, hipGraph-captured; against the plain per-product path on the
     same products (device-resident Plan.execute, plans made beforehand) and
-    the host one-shot path (plan, copies, execute, release) on a sample."""
+    the host one-shot path (plan, copies, execute, release) on a sample.
+    tune=False keeps the default splits and compare=False skips both plain
+    paths (a profiled run then traces the batch launches alone)."""
     import torch
     import birdwing_amd as bw
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -203,8 +205,9 @@ def bench_batch(n, steps, warmup, seed=4096, host_sample=128):
     dc = torch.zeros(b.bytes_c, dtype=torch.uint8, device="cuda")
     s = torch.cuda.Stream()
     t_tune = time.perf_counter()
-    with torch.cuda.stream(s):
-        b.tune(da, db, stream=s, reps=2)
+    if tune:
+        with torch.cuda.stream(s):
+            b.tune(da, db, stream=s, reps=2)
     t_tune = time.perf_counter() - t_tune
     b.set_graph(True)
     with torch.cuda.stream(s):
@@ -225,6 +228,28 @@ def bench_batch(n, steps, warmup, seed=4096, host_sample=128):
     out = dc.cpu().numpy()
     got = b.unpack(out)
     mbytes = sum(product_model_bytes(x, y, 32) for x, y in zip(LA, LB))
+    groups = b.groups()
+    res = {
+        "workload": "batch_%d_logu_2^8-2^16_u32" % n,
+        "products": n,
+        "steps": steps,
+        "products_per_s": n / batch_s,
+        "ms_per_batch": batch_s * 1e3,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "alg_bytes": mbytes,
+        "achieved": mbytes / batch_s / 1e9,
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": mbytes / batch_s / 1e9 / HBM_PEAK_GBPS,
+        "graph": True,
+        "launches": int(b.info.nlaunch),
+        "autotuned": tune,
+        "autotune_s": t_tune,
+        "groups": [{"logN": g["logN"], "members": g["members"], "passes": g["passes"], "gshift": g["gshift"]}
+                   for g in groups],
+    }
+    if not compare:
+        return res
 
     # plain per-product path on the same products, same device buffers
     plans = [bw.Plan(32, int(LA[i]), int(LB[i])) for i in range(n)]
@@ -261,31 +286,15 @@ def bench_batch(n, steps, warmup, seed=4096, host_sample=128):
             g, _ = pyoracle.gmp_mul(A, B, 32)
             exact = exact and bool(np.array_equal(g, r)) and bool(np.array_equal(g, got[i]))
     host_s = (time.perf_counter() - t4) / len(idx)
-    groups = b.groups()
-    return {
-        "workload": "batch_%d_logu_2^8-2^16_u32" % n,
-        "products": n,
-        "steps": steps,
-        "products_per_s": n / batch_s,
-        "ms_per_batch": batch_s * 1e3,
-        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-        "alg_bytes": mbytes,
-        "achieved": mbytes / batch_s / 1e9,
-        "peak": HBM_PEAK_GBPS,
-        "unit": "GB/s",
-        "frac": mbytes / batch_s / 1e9 / HBM_PEAK_GBPS,
+    res.update({
         "plain_device_products_per_s": 1.0 / plain_s,
         "plain_host_products_per_s": 1.0 / host_s,
         "vs_plain_device": plain_s / batch_s * n,
         "vs_plain_host": host_s / batch_s * n,
         "bit_exact_vs_gmp_sample": exact,
         "same_as_plain_sample": same,
-        "graph": True,
-        "launches": int(b.info.nlaunch),
-        "autotune_s": t_tune,
-        "groups": [{"logN": g["logN"], "members": g["members"], "passes": g["passes"], "gshift": g["gshift"]}
-                   for g in groups],
-    }
+    })
+    return res
 
 
 class _ServeMix:
@@ -464,6 +473,9 @@ def main():
     ap.add_argument("--no-oracle", action="store_true", help="GMP only as CPU baseline")
     ap.add_argument("--batch-n", type=int, default=4096, help="products in the batched workload (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-no-tune", action="store_true", help="batched workload with the default splits")
+    ap.add_argument("--batch-no-plain", action="store_true",
+                    help="skip the plain-path comparisons of the batched workload (profiling)")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="Poisson serving run length (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (HBM counter calibration) and exit")
@@ -589,7 +601,8 @@ def main():
     del plan, da, db, dc
     torch.cuda.empty_cache()
     if args.batch_n > 0:
-        line["batched"] = bench_batch(args.batch_n, args.batch_steps, 2)
+        line["batched"] = bench_batch(args.batch_n, args.batch_steps, 2, tune=not args.batch_no_tune,
+                                      compare=not args.batch_no_plain)
         torch.cuda.empty_cache()
     if args.serve_seconds > 0:
         line["served"] = bench_serve(args.serve_seconds)
