@@ -232,6 +232,30 @@ struct T2Lds {
     static constexpr int R1 = 1 << t2_r1log(LOGT, COLS);
     static constexpr int LT = (1 << LOGT) + (1 << LOGT) / R1 + 1;
     __device__ static int pos(int q) { return q + q / R1; }
+    // pos(k * R1 + j) for j < R1 (stage-0 element k of group j), with no division
+    __device__ static int pos_kj(int k, int j) { return k * (R1 + 1) + j; }
+};
+
+// 32-bit words of a tile through a raw buffer resource on the tile's first
+// element: each access adds a per-lane byte offset (VGPR) and a wave-uniform
+// byte offset (SGPR) in the memory instruction itself, so a strided sequence
+// of accesses costs no 64-bit vector address arithmetic.  Both offsets and
+// their sum stay below 2^32 (the tile spans less than 4 GiB).  No range
+// check: num_records is the maximum.
+struct TileBuf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ explicit TileBuf(const void* base)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, -1, 0x00020000)) {}
+    __device__ uint32_t ld(uint32_t lane_off, uint32_t uni_off) const {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, lane_off, uni_off, 0);
+    }
+    __device__ uint64_t ld64(uint32_t lane_off, uint32_t uni_off) const {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, lane_off, uni_off, 0);
+        return (uint64_t)v[0] | ((uint64_t)v[1] << 32);
+    }
+    __device__ void st(uint32_t v, uint32_t lane_off, uint32_t uni_off) const {
+        __builtin_amdgcn_raw_buffer_store_b32(v, r, lane_off, uni_off, 0);
+    }
 };
 // w_M^e from the two-level table, e < 2^table_log.
 // Fused pass: load both operands' stage-0 words before transforming A.
@@ -294,20 +318,31 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     // stage-0 group of this thread: line g0, offset j
     const int g0 = COLS ? (tid & (G - 1)) : (tid >> R1LOG);
     const int j0 = COLS ? (tid >> logG) : (tid & (R1 - 1));
+    // Column tiles of 32-bit words: element (g, t) of the tile sits at the
+    // wave-uniform tile0 plus g + t*S, below T*S <= 2^30 (the launcher
+    // checks), so accesses go through a TileBuf with 32-bit byte offsets.
+    constexpr bool LANE32 = COLS && sizeof(W) == 4;
+    const int64_t tile0 = o0 * T * S + s0;
+    // the column stride in bytes, pinned to a scalar register: every
+    // wave-uniform buffer offset is a multiple of it
+    const uint32_t sb = LANE32 ? (uint32_t)__builtin_amdgcn_readfirstlane((int)((uint32_t)S * (uint32_t)sizeof(W))) : 0u;
+    auto lane_bytes = [&](int g, int t) -> uint32_t { return (uint32_t)g * (uint32_t)sizeof(W) + (uint32_t)t * sb; };
 
     // raw limbs of operand `which` for this thread's line; `base` becomes the
     // position inside the product (stacked products: line o reads product o)
     // raw limb type read by ingress: a 64-bit digit is two 32-bit limbs
     using RL = typename std::conditional<INGRESS == 2, uint32_t, typename AR::Raw>::type;
-    auto ingress_src = [&](int which, const RL*& raw, int64_t& len, int64_t& base) {
+    // (column tiles: one product per tile, so raw, len and the shift are wave-uniform)
+    auto ingress_src = [&](int which, const RL*& raw, int64_t& len, int64_t& shift) {
         raw = reinterpret_cast<const RL*>(prm.in[which]);
         len = prm.in_len[which];
+        shift = 0;
         if (prm.bdesc) {
             const int64_t o = COLS ? o0 : o0 + g0;
             const BwIngress d = prm.bdesc[o];
             raw += which ? d.b_off : d.a_off;
             len = which ? d.b_len : d.a_len;
-            base -= o << prm.logM;
+            shift = o << prm.logM;
         }
     };
     // stage-0 input words of one operand (raw digits or residue words)
@@ -328,12 +363,40 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         const int64_t step = (int64_t)R1 * tstride;
         if constexpr (INGRESS != 0) {
             const RL* raw;
-            int64_t len;
-            ingress_src(which, raw, len, base);
+            int64_t len, shift;
+            ingress_src(which, raw, len, shift);
+            base -= shift;
             constexpr int LW = INGRESS == 2 ? 2 : 1;          // limbs per digit
             const int64_t last = base + (int64_t)(R0 - 1) * step;
             const bool upper_zero = R0 > 1 && LW * (base + (int64_t)(R0 / 2) * step) >= len;
-            if (LW * last + (LW - 1) < len) {
+            // LANE32: digit base + m*step = (tile0 - shift) + (g0 + j0*S) + m*step,
+            // the first term wave-uniform; byte offsets below 2^32 (T*S*8 <= 2^32)
+            const TileBuf rb(raw + LW * (tile0 - shift));
+            const uint32_t lb = LANE32 ? lane_bytes(g0, j0) * LW : 0u;
+            auto lane_load = [&](int mlo, int mhi) {
+                if constexpr (INGRESS == 2) {
+                    if ((reinterpret_cast<uintptr_t>(raw) & 7) == 0) {
+#pragma unroll
+                        for (int m = mlo; m < mhi; m++) w[m] = rb.ld64(lb, (uint32_t)(m * R1 * LW) * sb);
+                    } else {
+#pragma unroll
+                        for (int m = mlo; m < mhi; m++) {
+                            const uint32_t uo = (uint32_t)(m * R1 * LW) * sb;
+                            w[m] = (uint64_t)rb.ld(lb, uo) | ((uint64_t)rb.ld(lb + 4, uo) << 32);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int m = mlo; m < mhi; m++) w[m] = (IW)rb.ld(lb, (uint32_t)(m * R1 * LW) * sb);
+                }
+            };
+            if (LANE32 && LW * last + (LW - 1) < len) {
+                lane_load(0, R0);
+            } else if (LANE32 && upper_zero && LW * (base + (int64_t)(R0 / 2 - 1) * step) + (LW - 1) < len) {
+                lane_load(0, R0 / 2);
+#pragma unroll
+                for (int m = R0 / 2; m < R0; m++) w[m] = (IW)0;
+            } else if (LW * last + (LW - 1) < len) {
                 if constexpr (INGRESS == 2) {
                     if ((reinterpret_cast<uintptr_t>(raw) & 7) == 0) {
                         const uint64_t* r64 = reinterpret_cast<const uint64_t*>(raw);
@@ -389,6 +452,12 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
                 }
             }
             return upper_zero;
+        } else if constexpr (LANE32) {
+            const TileBuf in(reinterpret_cast<const W*>(prm.in[which]) + plane_off + tile0);
+            const uint32_t lb = lane_bytes(g0, j0);
+#pragma unroll
+            for (int m = 0; m < R0; m++) w[m] = (W)in.ld(lb, (uint32_t)(m * R1) * sb);
+            return false;
         } else {
             const W* in = reinterpret_cast<const W*>(prm.in[which]) + plane_off;
 #pragma unroll
@@ -421,7 +490,7 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         for (int k = 1; k < R0; k++) x[brev(k, R0LOG)] = AR::mul(x[brev(k, R0LOG)], twl[j0 * k], P);
 #pragma unroll
         for (int k = 0; k < R0; k++)   // k > 0 left a twiddle product in (-p, p)
-            lds[g0 * LT + T2Lds<LOGT, COLS>::pos(k * R1 + j0)] =
+            lds[g0 * LT + T2Lds<LOGT, COLS>::pos_kj(k, j0)] =
                 k == 0 ? AR::to_lds(x[0], P) : AR::to_lds_r(x[brev(k, R0LOG)]);
     };
     auto fwd_stage0 = [&](int which) {
@@ -467,6 +536,12 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
                 continue;
             }
             W* out = reinterpret_cast<W*>(prm.out[which]) + plane_off + line_base(g) + (int64_t)b * R1 * tstride;
+            const TileBuf out_u(reinterpret_cast<W*>(prm.out[which]) + plane_off + tile0);   // LANE32
+            const uint32_t lb = LANE32 ? lane_bytes(g, b * R1) : 0u;
+            auto put = [&](int k, W v) {
+                if constexpr (LANE32) out_u.st((uint32_t)v, lb, (uint32_t)k * sb);
+                else out[k * tstride] = v;
+            };
             if constexpr (COLS) {
                 const int64_t s = s0 + g;
                 typename AR::V tw[R1];
@@ -478,13 +553,13 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
             // one uniform branch around the stores: no per-element select of both forms
             if (canon) {
 #pragma unroll
-                for (int k = 0; k < R1; k++) out[k * tstride] = AR::to_word_c(x[brev(k, R1LOG)], P, true);
+                for (int k = 0; k < R1; k++) put(k, AR::to_word_c(x[brev(k, R1LOG)], P, true));
             } else if constexpr (COLS) {      // every value left a pass-twiddle product in (-p, p)
 #pragma unroll
-                for (int k = 0; k < R1; k++) out[k * tstride] = AR::to_word_r(x[brev(k, R1LOG)]);
+                for (int k = 0; k < R1; k++) put(k, AR::to_word_r(x[brev(k, R1LOG)]));
             } else {
 #pragma unroll
-                for (int k = 0; k < R1; k++) out[k * tstride] = AR::to_word_c(x[brev(k, R1LOG)], P, false);
+                for (int k = 0; k < R1; k++) put(k, AR::to_word_c(x[brev(k, R1LOG)], P, false));
             }
         }
     };
@@ -504,6 +579,12 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
                 for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::from_keep(keep[it * R1 + k]);
             } else {
                 const W* in = reinterpret_cast<const W*>(prm.in[0]) + plane_off + line_base(g) + (int64_t)b * R1 * tstride;
+                const TileBuf in_u(reinterpret_cast<const W*>(prm.in[0]) + plane_off + tile0);   // LANE32
+                const uint32_t lb = LANE32 ? lane_bytes(g, b * R1) : 0u;
+                auto get = [&](int k) -> W {
+                    if constexpr (LANE32) return (W)in_u.ld(lb, (uint32_t)k * sb);
+                    else return in[k * tstride];
+                };
                 if constexpr (COLS) {
                     const int64_t s = s0 + g;
                     const int64_t mask = ((int64_t)1 << prm.table_log) - 1;
@@ -513,7 +594,7 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
                     AR::template gen_powers<R1>(base, tw_lookup<AR>((-(((int64_t)R0 * s) << twshift)) & mask, twlo, twhi, pc),
                                                 tw, P);
 #pragma unroll
-                    for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::apply_tw(AR::from_word(in[k * tstride], P), tw[k], P);
+                    for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::apply_tw(AR::from_word(get(k), P), tw[k], P);
                 } else {
 #pragma unroll
                     for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::from_word(in[k], P);
@@ -535,16 +616,22 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     auto inv_stage0 = [&]() {
         V x[R0];
 #pragma unroll
-        for (int k = 0; k < R0; k++) x[brev(k, R0LOG)] = AR::from_lds(lds[g0 * LT + T2Lds<LOGT, COLS>::pos(k * R1 + j0)]);
+        for (int k = 0; k < R0; k++) x[brev(k, R0LOG)] = AR::from_lds(lds[g0 * LT + T2Lds<LOGT, COLS>::pos_kj(k, j0)]);
         dft32_dit_inv<AR, R0LOG>(x, pc);     // stage twiddles were applied before the transpose
         W* out = reinterpret_cast<W*>(prm.out[0]) + plane_off + line_base(g0) + j0 * tstride;
+        const TileBuf out_u(reinterpret_cast<W*>(prm.out[0]) + plane_off + tile0);    // LANE32
+        const uint32_t lb = LANE32 ? lane_bytes(g0, j0) : 0u;
         const int64_t step = (int64_t)R1 * tstride;
+        auto put = [&](int m, W v) {
+            if constexpr (LANE32) out_u.st((uint32_t)v, lb, (uint32_t)(m * R1) * sb);
+            else out[m * step] = v;
+        };
         if (canon) {
 #pragma unroll
-            for (int m = 0; m < R0; m++) out[m * step] = AR::to_word_c(x[m], P, true);
+            for (int m = 0; m < R0; m++) put(m, AR::to_word_c(x[m], P, true));
         } else {
 #pragma unroll
-            for (int m = 0; m < R0; m++) out[m * step] = AR::to_word_c(x[m], P, false);
+            for (int m = 0; m < R0; m++) put(m, AR::to_word_c(x[m], P, false));
         }
     };
 
