@@ -296,7 +296,16 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     // stage twiddles w_T^e (e < T) staged once per workgroup after the tile
     TW* twl = reinterpret_cast<TW*>(smem_raw + t2_tile_bytes(G, LT, (int)sizeof(typename AR::LV)));
     const int op = (MODE == BW_MODE_FWD) ? blockIdx.z : 0;
-    const int bx = blockIdx.x, prime = prm.prime0 + blockIdx.y;
+    // Workgroup order: the ingress pass walks the tiles of one prime after
+    // another (blockIdx.x fastest: the raw operands, shared by all primes,
+    // stream through in order once per prime); the other passes take the
+    // primes of a tile back to back, which measured 3-7% faster for them.
+    int bx = blockIdx.x, prime = prm.prime0 + blockIdx.y;
+    if constexpr (INGRESS == 0) {
+        const uint32_t id = blockIdx.x + blockIdx.y * gridDim.x;
+        bx = (int)(id / gridDim.y);
+        prime = prm.prime0 + (int)(id % gridDim.y);
+    }
     const typename AR::PC32 pc = reinterpret_cast<const typename AR::PC32*>(prm.consts)[prime];
     const auto& P = AR::prime(pc);
     const TW* tws = reinterpret_cast<const TW*>(prm.tw_stage) + ((int64_t)prime << BW_STAGE_LOG);
