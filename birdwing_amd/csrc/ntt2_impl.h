@@ -316,6 +316,16 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     const int64_t o0 = COLS ? l0 / S : l0;
     const int64_t s0 = COLS ? l0 - o0 * S : 0;
     const int twshift = prm.table_log - prm.logM;
+    // Forward column passes on the FP64 path take the pass-twiddle bases of
+    // column s = s0 + g from two LDS tables filled once per workgroup: w^(b*g)
+    // and w^(R0*g) from ztab (e < R0*G, the same for every tile) times
+    // w^(b*s0) and w^(R0*s0) from utab.  Looking them up per lane instead
+    // scatters every lane over the twiddle tables once M is large.  (The
+    // inverse passes keep the per-lane look-ups: the tables' LDS would cost
+    // them a workgroup per CU.)
+    constexpr bool CTAB = COLS && sizeof(W) == 4 && MODE == BW_MODE_FWD;
+    TW* ztab = twl + T;
+    TW* utab = ztab + (R0 << logG);
     const int64_t plane_off = (int64_t)prime * prm.plane;
     const int tid = threadIdx.x;
     const bool canon = prm.canon_out != 0;
@@ -554,8 +564,12 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
             if constexpr (COLS) {
                 const int64_t s = s0 + g;
                 typename AR::V tw[R1];
-                AR::template gen_powers<R1>(tw_lookup<AR>(((int64_t)b * s) << twshift, twlo, twhi, pc),
-                                            tw_lookup<AR>(((int64_t)R0 * s) << twshift, twlo, twhi, pc), tw, P);
+                if constexpr (CTAB)
+                    AR::template gen_powers<R1>(AR::twprod(utab[b], ztab[b * g], P), AR::twprod(utab[R0], ztab[R0 * g], P),
+                                                tw, P);
+                else
+                    AR::template gen_powers<R1>(tw_lookup<AR>(((int64_t)b * s) << twshift, twlo, twhi, pc),
+                                                tw_lookup<AR>(((int64_t)R0 * s) << twshift, twlo, twhi, pc), tw, P);
 #pragma unroll
                 for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::apply_tw(x[brev(k, R1LOG)], tw[k], P);
             }
@@ -574,16 +588,23 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     };
     // stage 1 inverse: keep (fused) or HBM with the inverse pass twiddle -> inverse DFT_R1
     // -> inverse stage twiddle -> LDS
+    // The inverse pass (IT1 == 1) waits for the stage-twiddle fill only here,
+    // after its loads and its DFT: every thread, idle slots included, reaches
+    // that barrier.
     auto inv_stage1 = [&](auto km) {
         constexpr bool KEEP = decltype(km)::value != 0;
+        constexpr bool LATE_SYNC = !KEEP && MODE == BW_MODE_INV && IT1 == 1;
 #pragma unroll
         for (int it = 0; it < IT1; it++) {
             const int gi = tid + it * NT;
-            if (IT1 * NT > G * R0 && gi >= G * R0) continue;   // idle stage-1 slot
+            const bool idle = IT1 * NT > G * R0 && gi >= G * R0;   // idle stage-1 slot
+            if (idle && !LATE_SYNC) continue;
             int g, b;
             dec1(gi, g, b);
             V x[R1];
-            if constexpr (KEEP) {
+            if (idle) {
+                // LATE_SYNC only: no work, but the barrier below
+            } else if constexpr (KEEP) {
 #pragma unroll
                 for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::from_keep(keep[it * R1 + k]);
             } else {
@@ -597,19 +618,25 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
                 if constexpr (COLS) {
                     const int64_t s = s0 + g;
                     const int64_t mask = ((int64_t)1 << prm.table_log) - 1;
-                    TW base = tw_lookup<AR>((-(((int64_t)b * s) << twshift)) & mask, twlo, twhi, pc);
+                    TW base = CTAB ? AR::twprod(utab[b], ztab[b * g], P)
+                                   : tw_lookup<AR>((-(((int64_t)b * s) << twshift)) & mask, twlo, twhi, pc);
                     if (prm.fold_ninv) base = AR::tw_mul_ninv(base, ninv, P);
+                    const TW step = CTAB ? AR::twprod(utab[R0], ztab[R0 * g], P)
+                                         : tw_lookup<AR>((-(((int64_t)R0 * s) << twshift)) & mask, twlo, twhi, pc);
                     typename AR::V tw[R1];
-                    AR::template gen_powers<R1>(base, tw_lookup<AR>((-(((int64_t)R0 * s) << twshift)) & mask, twlo, twhi, pc),
-                                                tw, P);
+                    AR::template gen_powers<R1>(base, step, tw, P);
 #pragma unroll
                     for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::apply_tw(AR::from_word(get(k), P), tw[k], P);
                 } else {
 #pragma unroll
-                    for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::from_word(in[k], P);
+                    for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::from_word(get(k), P);
                 }
             }
-            dft32_dit_inv<AR, R1LOG>(x, pc);
+            if (!idle) dft32_dit_inv<AR, R1LOG>(x, pc);
+            if constexpr (LATE_SYNC) {
+                __syncthreads();
+                if (idle) continue;
+            }
             // the inverse stage twiddle of position b*R1 + m (stage-0 group m,
             // index b) is applied here: its product leaves the value reduced.
             // m = 0 is untwiddled (a reduction); b = 0 multiplies by twl[0] = 1,
@@ -646,6 +673,15 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
 
     auto fill_twiddles = [&]() {
         for (int i = tid; i < T; i += NT) twl[i] = tws[i << TWSH];
+        if constexpr (CTAB) {
+            // conjugate powers for the inverse
+            const int64_t mask = ((int64_t)1 << prm.table_log) - 1;
+            const int64_t sg = MODE == BW_MODE_INV ? -1 : 1;
+            for (int i = tid; i < (R0 << logG); i += NT)
+                ztab[i] = tw_lookup<AR>((sg * ((int64_t)i << twshift)) & mask, twlo, twhi, pc);
+            for (int i = tid; i <= R0; i += NT)
+                utab[i] = tw_lookup<AR>((sg * (((int64_t)i * s0) << twshift)) & mask, twlo, twhi, pc);
+        }
     };
     if constexpr (MODE == BW_MODE_FWD) {
         IW w[R0];
@@ -657,6 +693,7 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         fwd_stage1(op, std::integral_constant<int, 0>());
     } else if constexpr (MODE == BW_MODE_INV) {
         fill_twiddles();
+        if constexpr (IT1 > 1) __syncthreads();   // else inside inv_stage1, after its loads
         inv_stage1(std::integral_constant<int, 0>());
         __syncthreads();
         inv_stage0();
