@@ -236,6 +236,19 @@ struct T2Lds {
     __device__ static int pos_kj(int k, int j) { return k * (R1 + 1) + j; }
 };
 
+// Element i of an array no kernel writes, read through the constant address
+// space (8-byte words): with a wave-uniform index the loads are scalar loads.
+template <class T>
+__device__ __forceinline__ T bw_ldconst(const T* p, int64_t i) {
+    static_assert(sizeof(T) % 8 == 0, "8-byte words");
+    const __attribute__((address_space(4))) uint64_t* q = (const __attribute__((address_space(4))) uint64_t*)(p + i);
+    T r;
+    uint64_t* d = reinterpret_cast<uint64_t*>(&r);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 8); k++) d[k] = q[k];
+    return r;
+}
+
 // 32-bit words of a tile through a raw buffer resource on the tile's first
 // element: each access adds a per-lane byte offset (VGPR) and a wave-uniform
 // byte offset (SGPR) in the memory instruction itself, so a strided sequence
@@ -277,7 +290,7 @@ __device__ __forceinline__ typename AR::TW tw_lookup(int64_t e, const typename A
 // limb per digit, 2: 64-bit digits from pairs of 32-bit limbs) are compile-time so
 // the inner loops carry no branches.
 template <class AR, int LOGT, int MODE, bool COLS, int INGRESS>
-__global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
+__device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int prime) {
     using V = typename AR::V;
     using W = typename AR::W;
     using TW = typename AR::TW;
@@ -296,17 +309,9 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
     // stage twiddles w_T^e (e < T) staged once per workgroup after the tile
     TW* twl = reinterpret_cast<TW*>(smem_raw + t2_tile_bytes(G, LT, (int)sizeof(typename AR::LV)));
     const int op = (MODE == BW_MODE_FWD) ? blockIdx.z : 0;
-    // Workgroup order: the ingress pass walks the tiles of one prime after
-    // another (blockIdx.x fastest: the raw operands, shared by all primes,
-    // stream through in order once per prime); the other passes take the
-    // primes of a tile back to back, which measured 3-7% faster for them.
-    int bx = blockIdx.x, prime = prm.prime0 + blockIdx.y;
-    if constexpr (INGRESS == 0) {
-        const uint32_t id = blockIdx.x + blockIdx.y * gridDim.x;
-        bx = (int)(id / gridDim.y);
-        prime = prm.prime0 + (int)(id % gridDim.y);
-    }
-    const typename AR::PC32 pc = reinterpret_cast<const typename AR::PC32*>(prm.consts)[prime];
+    // read-only inputs of the launch through the constant address space, so
+    // they load as scalars however the kernel's stores might alias them
+    const typename AR::PC32 pc = bw_ldconst(reinterpret_cast<const typename AR::PC32*>(prm.consts), prime);
     const auto& P = AR::prime(pc);
     const TW* tws = reinterpret_cast<const TW*>(prm.tw_stage) + ((int64_t)prime << BW_STAGE_LOG);
     const TW* twlo = reinterpret_cast<const TW*>(prm.tw_lo) + ((int64_t)prime << BW_TW_LO_LOG);
@@ -358,7 +363,7 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         shift = 0;
         if (prm.bdesc) {
             const int64_t o = COLS ? o0 : o0 + g0;
-            const BwIngress d = prm.bdesc[o];
+            const BwIngress d = bw_ldconst(prm.bdesc, o);
             raw += which ? d.b_off : d.a_off;
             len = which ? d.b_len : d.a_len;
             shift = o << prm.logM;
@@ -730,6 +735,21 @@ __global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
         inv_stage1(std::integral_constant<int, 1>());
         __syncthreads();
         inv_stage0();
+    }
+}
+
+// The kernel: which tile and prime a workgroup takes.  The ingress pass
+// walks the tiles of one prime after another (blockIdx.x fastest: the raw
+// operands, read by every prime, stream through in order once per prime);
+// the other passes take the primes of a tile back to back, which measured
+// 3-7% faster for them.
+template <class AR, int LOGT, int MODE, bool COLS, int INGRESS>
+__global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
+    if constexpr (INGRESS == 0) {
+        const uint32_t id = blockIdx.x + blockIdx.y * gridDim.x;
+        ntt2_body<AR, LOGT, MODE, COLS, INGRESS>(prm, (int)(id / gridDim.y), prm.prime0 + (int)(id % gridDim.y));
+    } else {
+        ntt2_body<AR, LOGT, MODE, COLS, INGRESS>(prm, blockIdx.x, prm.prime0 + blockIdx.y);
     }
 }
 
