@@ -45,13 +45,49 @@ __device__ __forceinline__ double f64_reduce(double x, const F64Prime& P) {
     return __builtin_fma(-q, P.p, x);
 }
 
-// Canonical representative in [0, p).
-__device__ __forceinline__ uint32_t f64_canon(double x, const F64Prime& P) {
-    double r = f64_reduce(x, P);
-    r = r < 0.0 ? r + P.p : r;
-    r = r >= P.p ? r - P.p : r;
-    return (uint32_t)r;
+// Canonical representative in [0, p), as a double: the reduction leaves
+// |r| <= p/2 + 1 (q is the rounded quotient up to an error far below one
+// unit for |x| < 2^51), so adding p to a negative r is the only correction.
+__device__ __forceinline__ double f64_canon_d(double x, const F64Prime& P) {
+    const double r = f64_reduce(x, P);
+    return r < 0.0 ? r + P.p : r;
 }
+__device__ __forceinline__ uint32_t f64_canon(double x, const F64Prime& P) { return (uint32_t)f64_canon_d(x, P); }
+
+// Element i of an array no kernel writes, read through the constant address
+// space (8-byte words): with a wave-uniform index the loads are scalar loads.
+template <class T>
+__device__ __forceinline__ T bw_ldconst(const T* p, int64_t i) {
+    static_assert(sizeof(T) % 8 == 0, "8-byte words");
+    const __attribute__((address_space(4))) uint64_t* q = (const __attribute__((address_space(4))) uint64_t*)(p + i);
+    T r;
+    uint64_t* d = reinterpret_cast<uint64_t*>(&r);
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 8); k++) d[k] = q[k];
+    return r;
+}
+
+// 32-bit words of a tile through a raw buffer resource on the tile's first
+// element: each access adds a per-lane byte offset (VGPR) and a wave-uniform
+// byte offset (SGPR) in the memory instruction itself, so a strided sequence
+// of accesses costs no 64-bit vector address arithmetic.  Both offsets and
+// their sum stay below 2^32 (the tile spans less than 4 GiB).  No range
+// check: num_records is the maximum.
+struct TileBuf {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ explicit TileBuf(const void* base)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, -1, 0x00020000)) {}
+    __device__ uint32_t ld(uint32_t lane_off, uint32_t uni_off) const {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, lane_off, uni_off, 0);
+    }
+    __device__ uint64_t ld64(uint32_t lane_off, uint32_t uni_off) const {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, lane_off, uni_off, 0);
+        return (uint64_t)v[0] | ((uint64_t)v[1] << 32);
+    }
+    __device__ void st(uint32_t v, uint32_t lane_off, uint32_t uni_off) const {
+        __builtin_amdgcn_raw_buffer_store_b32(v, r, lane_off, uni_off, 0);
+    }
+};
 
 // ---------------------------------------------------------------------------
 // 64-bit path.

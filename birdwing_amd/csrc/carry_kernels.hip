@@ -51,15 +51,17 @@ __device__ __forceinline__ void garner_u64(uint64_t r0, uint64_t r1, uint64_t r2
 //   k_j = (...((r_j - k_0) p_0^-1 - k_1) p_1^-1 ... - k_{j-1}) p_{j-1}^-1 mod p_j
 // in FP64 products, then x = k_0 + p_0 (k_1 + p_1 (k_2 + p_2 (k_3 + p_3 k_4)))
 // by Horner steps on 32-bit words; x < 2^154 -> three 64-bit words.
-__device__ __forceinline__ void garner_d64(const uint32_t* r, int64_t plane, int64_t n, const GarnerD64Dev& g,
-                                           uint64_t* w) {
+// r: the five residue words of one digit (canonical or centered residues).
+__device__ __forceinline__ void garner_d64(const uint32_t* r, const GarnerD64Dev& g, uint64_t* w) {
     uint32_t k[5];
+    double kd[5];      // the same digits as doubles (no conversion back)
 #pragma unroll
     for (int j = 0; j < 5; j++) {
-        double t = (double)(int32_t)r[j * plane + n];   // canonical or centered residue
+        double t = (double)(int32_t)r[j];
 #pragma unroll
-        for (int i = 0; i < j; i++) t = f64_mulmod(t - (double)k[i], g.inv[j][i][0], g.inv[j][i][1], g.fp[j]);
-        k[j] = f64_canon(t, g.fp[j]);
+        for (int i = 0; i < j; i++) t = f64_mulmod(t - kd[i], g.inv[j][i][0], g.inv[j][i][1], g.fp[j]);
+        kd[j] = f64_canon_d(t, g.fp[j]);
+        k[j] = (uint32_t)kd[j];
     }
     uint32_t x[5] = {k[4], 0u, 0u, 0u, 0u};
 #pragma unroll
@@ -115,7 +117,11 @@ struct SrcGarnerD64 {
     static constexpr int NW = 3;
     __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
         const bool in = n >= 0 && n < p.N;
-        garner_d64(reinterpret_cast<const uint32_t*>(p.a0), p.plane, in ? n : 0, c.gd, w);
+        const uint32_t* a = reinterpret_cast<const uint32_t*>(p.a0) + (in ? n : 0);
+        uint32_t r[5];
+#pragma unroll
+        for (int j = 0; j < 5; j++) r[j] = a[j * p.plane];
+        garner_d64(r, c.gd, w);
         if (!in) w[0] = w[1] = w[2] = 0;
     }
 };

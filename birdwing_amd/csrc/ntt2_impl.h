@@ -236,40 +236,7 @@ struct T2Lds {
     __device__ static int pos_kj(int k, int j) { return k * (R1 + 1) + j; }
 };
 
-// Element i of an array no kernel writes, read through the constant address
-// space (8-byte words): with a wave-uniform index the loads are scalar loads.
-template <class T>
-__device__ __forceinline__ T bw_ldconst(const T* p, int64_t i) {
-    static_assert(sizeof(T) % 8 == 0, "8-byte words");
-    const __attribute__((address_space(4))) uint64_t* q = (const __attribute__((address_space(4))) uint64_t*)(p + i);
-    T r;
-    uint64_t* d = reinterpret_cast<uint64_t*>(&r);
-#pragma unroll
-    for (int k = 0; k < (int)(sizeof(T) / 8); k++) d[k] = q[k];
-    return r;
-}
 
-// 32-bit words of a tile through a raw buffer resource on the tile's first
-// element: each access adds a per-lane byte offset (VGPR) and a wave-uniform
-// byte offset (SGPR) in the memory instruction itself, so a strided sequence
-// of accesses costs no 64-bit vector address arithmetic.  Both offsets and
-// their sum stay below 2^32 (the tile spans less than 4 GiB).  No range
-// check: num_records is the maximum.
-struct TileBuf {
-    __amdgpu_buffer_rsrc_t r;
-    __device__ explicit TileBuf(const void* base)
-        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, -1, 0x00020000)) {}
-    __device__ uint32_t ld(uint32_t lane_off, uint32_t uni_off) const {
-        return __builtin_amdgcn_raw_buffer_load_b32(r, lane_off, uni_off, 0);
-    }
-    __device__ uint64_t ld64(uint32_t lane_off, uint32_t uni_off) const {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, lane_off, uni_off, 0);
-        return (uint64_t)v[0] | ((uint64_t)v[1] << 32);
-    }
-    __device__ void st(uint32_t v, uint32_t lane_off, uint32_t uni_off) const {
-        __builtin_amdgcn_raw_buffer_store_b32(v, r, lane_off, uni_off, 0);
-    }
-};
 // w_M^e from the two-level table, e < 2^table_log.
 // Fused pass: load both operands' stage-0 words before transforming A.
 #ifndef BW_FUSED_PREFETCH
