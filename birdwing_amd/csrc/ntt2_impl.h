@@ -292,10 +292,12 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
     // column s = s0 + g from two LDS tables filled once per workgroup: w^(b*g)
     // and w^(R0*g) from ztab (e < R0*G, the same for every tile) times
     // w^(b*s0) and w^(R0*s0) from utab.  Looking them up per lane instead
-    // scatters every lane over the twiddle tables once M is large.  (The
-    // inverse passes keep the per-lane look-ups: the tables' LDS would cost
-    // them a workgroup per CU.)
-    constexpr bool CTAB = COLS && sizeof(W) == 4 && MODE == BW_MODE_FWD;
+    // scatters every lane over the twiddle tables once M is large.  Tiles of
+    // R1 * R1 points only (T = 256): larger tiles would lose a workgroup per
+    // CU to the tables' LDS, and the smaller tiles of the batched transforms
+    // (small M, look-ups that stay in L1) measured 1-2% slower with them; the
+    // inverse passes keep the per-lane look-ups too (LDS occupancy).
+    constexpr bool CTAB = COLS && sizeof(W) == 4 && MODE == BW_MODE_FWD && R0 == R1;
     TW* ztab = twl + T;
     TW* utab = ztab + (R0 << logG);
     const int64_t plane_off = (int64_t)prime * prm.plane;
