@@ -312,8 +312,9 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
     const int g0 = COLS ? (tid & (G - 1)) : (tid >> R1LOG);
     const int j0 = COLS ? (tid >> logG) : (tid & (R1 - 1));
     // Column tiles of 32-bit words: element (g, t) of the tile sits at the
-    // wave-uniform tile0 plus g + t*S, below T*S <= 2^30 (the launcher
-    // checks), so accesses go through a TileBuf with 32-bit byte offsets.
+    // wave-uniform tile0 plus g + t*S, below T*S <= 2^29 (the launcher
+    // checks; 8 bytes per raw digit of the ingress pass still stay below
+    // 2^32), so accesses go through a TileBuf with 32-bit byte offsets.
     constexpr bool LANE32 = COLS && sizeof(W) == 4;
     const int64_t tile0 = o0 * T * S + s0;
     // the column stride in bytes, pinned to a scalar register: every
@@ -321,11 +322,12 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
     const uint32_t sb = LANE32 ? (uint32_t)__builtin_amdgcn_readfirstlane((int)((uint32_t)S * (uint32_t)sizeof(W))) : 0u;
     auto lane_bytes = [&](int g, int t) -> uint32_t { return (uint32_t)g * (uint32_t)sizeof(W) + (uint32_t)t * sb; };
 
-    // raw limbs of operand `which` for this thread's line; `base` becomes the
-    // position inside the product (stacked products: line o reads product o)
-    // raw limb type read by ingress: a 64-bit digit is two 32-bit limbs
+    // raw limbs of operand `which` for this thread's line, their count, and
+    // the shift that turns a line position into a position inside the
+    // product (stacked products: line o reads product o, shift o * M); the
+    // raw limb type: a 64-bit digit is two 32-bit limbs.  Column tiles hold
+    // one product per tile, so raw, len and the shift are wave-uniform.
     using RL = typename std::conditional<INGRESS == 2, uint32_t, typename AR::Raw>::type;
-    // (column tiles: one product per tile, so raw, len and the shift are wave-uniform)
     auto ingress_src = [&](int which, const RL*& raw, int64_t& len, int64_t& shift) {
         raw = reinterpret_cast<const RL*>(prm.in[which]);
         len = prm.in_len[which];
