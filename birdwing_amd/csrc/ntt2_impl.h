@@ -103,6 +103,24 @@ struct ArF64x {
 #pragma unroll
         for (int k = 4; k < NTW; k++) w[k] = f64_mulmod(w[k - 4], s4, s4p, P);
     }
+    // the same twiddles handed to f(k, w_k) as the chains produce them: only
+    // the four chain heads stay live (fewer registers than an array of NTW)
+    template <int NTW, class F>
+    __device__ static void for_powers(TW base, TW step, const PR& P, F&& f) {
+        const double s = step.x, sp = step.y;
+        const double s2 = f64_mulmod(s, s, sp, P), s2p = s2 * P.pinv;
+        const double s4 = f64_mulmod(s2, s2, s2p, P), s4p = s4 * P.pinv;
+        double c[4];
+        c[0] = base.x;
+        if constexpr (NTW > 1) c[1] = f64_mulmod(base.x, s, sp, P);
+        if constexpr (NTW > 2) c[2] = f64_mulmod(base.x, s2, s2p, P);
+        if constexpr (NTW > 3) c[3] = f64_mulmod(c[1], s2, s2p, P);
+#pragma unroll
+        for (int k = 0; k < NTW; k++) {
+            f(k, c[k & 3]);
+            if (k + 4 < NTW) c[k & 3] = f64_mulmod(c[k & 3], s4, s4p, P);
+        }
+    }
     __device__ static V apply_tw(V x, double w, const PR& P) { return f64_mulmod(x, w, w * P.pinv, P); }
     __device__ static const PR& prime(const PC32& c) { return c.P; }
 };
@@ -174,6 +192,21 @@ struct ArU64x {
         if constexpr (NTW > 3) w[3] = u64_montmul(w[1], s2, P);
 #pragma unroll
         for (int k = 4; k < NTW; k++) w[k] = u64_montmul(w[k - 4], s4, P);
+    }
+    template <int NTW, class F>
+    __device__ static void for_powers(TW base, TW step, const PR& P, F&& f) {
+        const uint64_t s = u64_montmul(step.x, P.r2, P);
+        const uint64_t s2 = u64_montmul(s, s, P), s4 = u64_montmul(s2, s2, P);
+        uint64_t c[4];
+        c[0] = u64_montmul(base.x, P.r2, P);
+        if constexpr (NTW > 1) c[1] = u64_montmul(c[0], s, P);
+        if constexpr (NTW > 2) c[2] = u64_montmul(c[0], s2, P);
+        if constexpr (NTW > 3) c[3] = u64_montmul(c[1], s2, P);
+#pragma unroll
+        for (int k = 0; k < NTW; k++) {
+            f(k, c[k & 3]);
+            if (k + 4 < NTW) c[k & 3] = u64_montmul(c[k & 3], s4, P);
+        }
     }
     __device__ static V apply_tw(V x, uint64_t w, const PR& P) { return u64_montmul(x, w, P); }
     __device__ static const PR& prime(const PC32& c) { return c.P; }
@@ -288,16 +321,16 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
     const int64_t o0 = COLS ? l0 / S : l0;
     const int64_t s0 = COLS ? l0 - o0 * S : 0;
     const int twshift = prm.table_log - prm.logM;
-    // Forward column passes on the FP64 path take the pass-twiddle bases of
-    // column s = s0 + g from two LDS tables filled once per workgroup: w^(b*g)
-    // and w^(R0*g) from ztab (e < R0*G, the same for every tile) times
-    // w^(b*s0) and w^(R0*s0) from utab.  Looking them up per lane instead
-    // scatters every lane over the twiddle tables once M is large.  Tiles of
-    // R1 * R1 points only (T = 256): larger tiles would lose a workgroup per
-    // CU to the tables' LDS, and the smaller tiles of the batched transforms
-    // (small M, look-ups that stay in L1) measured 1-2% slower with them; the
-    // inverse passes keep the per-lane look-ups too (LDS occupancy).
-    constexpr bool CTAB = COLS && sizeof(W) == 4 && MODE == BW_MODE_FWD && R0 == R1;
+    // The forward ingress pass of 256-point column tiles on the FP64 path
+    // takes the pass-twiddle bases of column s = s0 + g from two LDS tables
+    // filled once per workgroup: w^(b*g) and w^(R0*g) from ztab (e < R0*G, the
+    // same for every tile) times w^(b*s0) and w^(R0*s0) from utab.  Looked up
+    // per lane, they scatter every lane over the twiddle tables once M is
+    // large (the ingress pass has M = N).  The other passes keep the per-lane
+    // look-ups: the tables' LDS costs them waves per SIMD (measured: the
+    // second forward pass of 2^24 points 5% slower with them, the larger
+    // tiles lose a workgroup per CU, the small batched tiles 1-2%).
+    constexpr bool CTAB = COLS && sizeof(W) == 4 && MODE == BW_MODE_FWD && R0 == R1 && INGRESS != 0;
     TW* ztab = twl + T;
     TW* utab = ztab + (R0 << logG);
     const int64_t plane_off = (int64_t)prime * prm.plane;
@@ -539,15 +572,13 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
             };
             if constexpr (COLS) {
                 const int64_t s = s0 + g;
-                typename AR::V tw[R1];
+                auto twx = [&](int k, auto w) { x[brev(k, R1LOG)] = AR::apply_tw(x[brev(k, R1LOG)], w, P); };
                 if constexpr (CTAB)
-                    AR::template gen_powers<R1>(AR::twprod(utab[b], ztab[b * g], P), AR::twprod(utab[R0], ztab[R0 * g], P),
-                                                tw, P);
+                    AR::template for_powers<R1>(AR::twprod(utab[b], ztab[b * g], P), AR::twprod(utab[R0], ztab[R0 * g], P),
+                                                P, twx);
                 else
-                    AR::template gen_powers<R1>(tw_lookup<AR>(((int64_t)b * s) << twshift, twlo, twhi, pc),
-                                                tw_lookup<AR>(((int64_t)R0 * s) << twshift, twlo, twhi, pc), tw, P);
-#pragma unroll
-                for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::apply_tw(x[brev(k, R1LOG)], tw[k], P);
+                    AR::template for_powers<R1>(tw_lookup<AR>(((int64_t)b * s) << twshift, twlo, twhi, pc),
+                                                tw_lookup<AR>(((int64_t)R0 * s) << twshift, twlo, twhi, pc), P, twx);
             }
             // one uniform branch around the stores: no per-element select of both forms
             if (canon) {
@@ -599,10 +630,9 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
                     if (prm.fold_ninv) base = AR::tw_mul_ninv(base, ninv, P);
                     const TW step = CTAB ? AR::twprod(utab[R0], ztab[R0 * g], P)
                                          : tw_lookup<AR>((-(((int64_t)R0 * s) << twshift)) & mask, twlo, twhi, pc);
-                    typename AR::V tw[R1];
-                    AR::template gen_powers<R1>(base, step, tw, P);
-#pragma unroll
-                    for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::apply_tw(AR::from_word(get(k), P), tw[k], P);
+                    AR::template for_powers<R1>(base, step, P, [&](int k, auto w) {
+                        x[brev(k, R1LOG)] = AR::apply_tw(AR::from_word(get(k), P), w, P);
+                    });
                 } else {
 #pragma unroll
                     for (int k = 0; k < R1; k++) x[brev(k, R1LOG)] = AR::from_word(get(k), P);

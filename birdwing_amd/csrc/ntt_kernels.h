@@ -91,18 +91,18 @@ __host__ __device__ constexpr int row_fused_lds_bytes(int logT) { return row_fus
 hipError_t bw_launch_row_fused(int logT, int64_t rows, int nprimes, const PassParams& prm, hipStream_t stream);
 // LDS of a register-staged workgroup: the tile (G lines of LT values of vbytes,
 // rounded up to 16 bytes), then the T stage twiddles (16-byte pairs), then
-// for forward column tiles of R1 * R1 points on the FP64 path (vbytes 4) the
-// column twiddle tables: R0*G powers w^e and R0 + 1 powers w^(b*s0) (16-byte
-// pairs).
+// for the forward ingress pass of R1 * R1-point column tiles on the FP64
+// path (vbytes 4) the column twiddle tables: R0*G powers w^e and R0 + 1
+// powers w^(b*s0) (16-byte pairs).
 __host__ __device__ constexpr int t2_tile_bytes(int G, int lt, int vbytes) { return (G * lt * vbytes + 15) / 16 * 16; }
-__host__ __device__ constexpr int t2_coltab_entries(int logT, bool cols, int G, int vbytes, int mode) {
-    return (cols && vbytes == 4 && mode == BW_MODE_FWD && t2_r0log(logT, cols) == t2_r1log(logT, cols))
+__host__ __device__ constexpr int t2_coltab_entries(int logT, bool cols, int G, int vbytes, int mode, int ingress) {
+    return (cols && vbytes == 4 && mode == BW_MODE_FWD && ingress != 0 && t2_r0log(logT, cols) == t2_r1log(logT, cols))
                ? (G << t2_r0log(logT, cols)) + (1 << t2_r0log(logT, cols)) + 1
                : 0;
 }
-__host__ __device__ constexpr int t2_lds_bytes(int logT, bool cols, int G, int vbytes, int mode) {
+__host__ __device__ constexpr int t2_lds_bytes(int logT, bool cols, int G, int vbytes, int mode, int ingress) {
     return t2_tile_bytes(G, t2_lds_line(logT, cols), vbytes) + (16 << logT) +
-           16 * t2_coltab_entries(logT, cols, G, vbytes, mode);
+           16 * t2_coltab_entries(logT, cols, G, vbytes, mode, ingress);
 }
 // Frequency held at position q of a pass line (either kernel family).
 __host__ __device__ inline int64_t bw_pass_freq_of_pos(int logT, int64_t q, bool cols) {

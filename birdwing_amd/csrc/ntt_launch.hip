@@ -64,7 +64,7 @@ hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_
     if (!k) return hipErrorInvalidValue;
     // LDS element: 8 bytes, except the register-staged 32-bit path (centered int32)
     const size_t vbytes = (reg2 && L.limb_bits == 32) ? 4 : 8;
-    const size_t lds = reg2 ? (size_t)t2_lds_bytes(L.logT, prm.S > 1, L.G, (int)vbytes, L.mode)
+    const size_t lds = reg2 ? (size_t)t2_lds_bytes(L.logT, prm.S > 1, L.G, (int)vbytes, L.mode, prm.ingress)
                             : (size_t)L.G * lds_line(L.logT) * vbytes;
     if (L.G > bw_pass_max_G(L.logT, L.limb_bits, prm.S > 1) || (L.G & (L.G - 1))) return hipErrorInvalidValue;
     if (prm.bdesc && prm.ingress && !reg2) return hipErrorInvalidValue;   // stacked ingress: reg2 only
@@ -79,7 +79,7 @@ hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_
         std::lock_guard<std::mutex> lock(mu);
         if (!done.count(reinterpret_cast<const void*>(k))) {
             const int gmax = bw_pass_max_G(L.logT, L.limb_bits, prm.S > 1);
-            const size_t lds_max = reg2 ? (size_t)t2_lds_bytes(L.logT, prm.S > 1, gmax, (int)vbytes, L.mode)
+            const size_t lds_max = reg2 ? (size_t)t2_lds_bytes(L.logT, prm.S > 1, gmax, (int)vbytes, L.mode, prm.ingress)
                                         : (size_t)gmax * lds_line(L.logT) * vbytes;
             hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max);
