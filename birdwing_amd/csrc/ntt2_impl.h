@@ -745,7 +745,12 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
 // the other passes take the primes of a tile back to back, which measured
 // 3-7% faster for them.
 template <class AR, int LOGT, int MODE, bool COLS, int INGRESS>
-__global__ __launch_bounds__(512) void ntt2_kernel(PassParams prm) {
+constexpr int ntt2_max_threads() {
+    return (MODE == BW_MODE_INV && COLS && t2_inv_wide_ok(LOGT, (int)sizeof(typename AR::LV))) ? 1024 : 512;
+}
+
+template <class AR, int LOGT, int MODE, bool COLS, int INGRESS>
+__global__ __launch_bounds__((ntt2_max_threads<AR, LOGT, MODE, COLS, INGRESS>())) void ntt2_kernel(PassParams prm) {
     if constexpr (INGRESS == 0) {
         const uint32_t id = blockIdx.x + blockIdx.y * gridDim.x;
         ntt2_body<AR, LOGT, MODE, COLS, INGRESS>(prm, (int)(id / gridDim.y), prm.prime0 + (int)(id % gridDim.y));

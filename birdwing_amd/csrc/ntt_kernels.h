@@ -104,6 +104,16 @@ __host__ __device__ constexpr int t2_lds_bytes(int logT, bool cols, int G, int v
     return t2_tile_bytes(G, t2_lds_line(logT, cols), vbytes) + (16 << logT) +
            16 * t2_coltab_entries(logT, cols, G, vbytes, mode, ingress);
 }
+// Inverse column passes of at most 256 points (R1 = 16) on the FP64 path run
+// 64-column tiles -- 256-byte row segments, 1024-thread workgroups -- when
+// their row stride is at least 2^BW_INV_WIDE_MIN_LOGS elements: with 128-byte
+// segments at a 256 KiB stride (the last inverse pass of 2^24 points) every
+// segment lands on its own DRAM page, and the pass ran 13% slower; at short
+// strides the wider workgroup measured 3% slower.
+#define BW_INV_WIDE_MIN_LOGS 12
+__host__ __device__ constexpr bool t2_inv_wide_ok(int logT, int vbytes) {
+    return vbytes == 4 && logT <= 8 && t2_r1log(logT, true) == 4;
+}
 // Frequency held at position q of a pass line (either kernel family).
 __host__ __device__ inline int64_t bw_pass_freq_of_pos(int logT, int64_t q, bool cols) {
     if (logT <= BW_REG2_MAX_LOG) {

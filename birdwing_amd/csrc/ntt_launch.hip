@@ -64,7 +64,16 @@ hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_
     if (!k) return hipErrorInvalidValue;
     // LDS element: 8 bytes, except the register-staged 32-bit path (centered int32)
     const size_t vbytes = (reg2 && L.limb_bits == 32) ? 4 : 8;
-    const size_t lds = reg2 ? (size_t)t2_lds_bytes(L.logT, prm.S > 1, L.G, (int)vbytes, L.mode, prm.ingress)
+    // long-stride inverse column passes: twice the tile width (the kernel's
+    // tile and prime mapping follow blockDim)
+    const bool wide_ok = reg2 && L.mode == BW_MODE_INV && prm.S > 1 && t2_inv_wide_ok(L.logT, (int)vbytes);
+    PassLaunch LW = L;
+    if (wide_ok && L.G == (512 >> 4) && prm.S >= ((int64_t)1 << BW_INV_WIDE_MIN_LOGS) && prm.S % (2 * L.G) == 0 &&
+        L.tiles % 2 == 0) {
+        LW.G = 2 * L.G;
+        LW.tiles = L.tiles / 2;
+    }
+    const size_t lds = reg2 ? (size_t)t2_lds_bytes(L.logT, prm.S > 1, LW.G, (int)vbytes, L.mode, prm.ingress)
                             : (size_t)L.G * lds_line(L.logT) * vbytes;
     if (L.G > bw_pass_max_G(L.logT, L.limb_bits, prm.S > 1) || (L.G & (L.G - 1))) return hipErrorInvalidValue;
     if (prm.bdesc && prm.ingress && !reg2) return hipErrorInvalidValue;   // stacked ingress: reg2 only
@@ -78,7 +87,7 @@ hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_
         static std::unordered_set<const void*> done;
         std::lock_guard<std::mutex> lock(mu);
         if (!done.count(reinterpret_cast<const void*>(k))) {
-            const int gmax = bw_pass_max_G(L.logT, L.limb_bits, prm.S > 1);
+            const int gmax = bw_pass_max_G(L.logT, L.limb_bits, prm.S > 1) * (wide_ok ? 2 : 1);
             const size_t lds_max = reg2 ? (size_t)t2_lds_bytes(L.logT, prm.S > 1, gmax, (int)vbytes, L.mode, prm.ingress)
                                         : (size_t)gmax * lds_line(L.logT) * vbytes;
             hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(k),
@@ -87,10 +96,10 @@ hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_
             done.insert(reinterpret_cast<const void*>(k));
         }
     }
-    dim3 grid((unsigned)L.tiles, (unsigned)L.nprimes, (unsigned)(L.mode == BW_MODE_FWD ? L.nops : 1));
+    dim3 grid((unsigned)LW.tiles, (unsigned)L.nprimes, (unsigned)(L.mode == BW_MODE_FWD ? L.nops : 1));
     // register-staged kernels run G * R1 threads (one stage-0 group each)
-    const unsigned nt = reg2 ? (unsigned)(L.G << t2_r1log(L.logT, prm.S > 1)) : 256u;
-    if (nt > 512 || nt == 0) return hipErrorInvalidValue;
+    const unsigned nt = reg2 ? (unsigned)(LW.G << t2_r1log(L.logT, prm.S > 1)) : 256u;
+    if (nt > (wide_ok ? 1024u : 512u) || nt == 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k, grid, dim3(nt), lds, stream, prm);
     return hipGetLastError();
 }

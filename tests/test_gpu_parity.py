@@ -261,6 +261,24 @@ def test_product_vs_gmp(gpu, oracle, LA, LB, bits):
     _product_check(gpu, oracle, A, B, bits)
 
 
+# Long-stride inverse column passes run 64-column tiles in 1024-thread
+# workgroups (row stride >= 2^12 elements, tiles of at most 256 points on the
+# FP64 path); the first pass of each split below qualifies: 256-point tiles,
+# 128-point tiles (half the threads idle in stage 1) and 64-point tiles.
+@pytest.mark.parametrize("bits,LA,LB,passes", [
+    (32, 1 << 20, 1 << 20, [8, 6, 6]),          # N = 2^20 64-bit digits, first-pass stride 2^12
+    (32, 1 << 19, (1 << 19) - 3, [7, 6, 6]),    # N = 2^19, stride 2^12, 128-point tiles
+    (32, 1 << 19, 1 << 19, [6, 7, 6]),          # N = 2^19, stride 2^13, 64-point tiles
+    (64, 1 << 18, (1 << 18) + 1, [7, 6, 6]),    # 64-bit limbs, N = 2^19, stride 2^12
+])
+def test_long_stride_inverse_tiles(gpu, oracle, bits, LA, LB, passes):
+    A = oracle.uniform_limbs(LA, LA + 91, bits)
+    B = oracle.uniform_limbs(LB, LB + 92, bits)
+    print("long-stride inverse", bits, LA, LB, passes, flush=True)
+    plan = _product_check(gpu, oracle, A, B, bits, -1, passes)
+    assert plan.passes == passes
+
+
 def test_product_reference_simple_cases(gpu, oracle):
     """tests/test_full_multiply.cpp:271-351: 1x1, 0x42, root probe, identities."""
     one = np.array([1, 0, 0, 0], np.uint32)
