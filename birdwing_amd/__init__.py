@@ -165,6 +165,23 @@ def _ptr(x):
     return x.ctypes.data_as(C.c_void_p)
 
 
+def _dev(x, nbytes, what):
+    """Address of a device buffer that holds at least `nbytes` bytes.
+
+    Device entry points take torch tensors on a HIP device, contiguous, large
+    enough for what the call reads or writes; anything else (a host array, a
+    strided view, a short tensor) would be an out-of-bounds device access, so
+    it is refused here, before the C ABI is called."""
+    if x is None or not hasattr(x, "data_ptr") or not getattr(x, "is_cuda", False):
+        raise BWError(BW_ERR_ARG, "%s must be a torch tensor on a HIP device" % what)
+    if not x.is_contiguous():
+        raise BWError(BW_ERR_ARG, "%s must be contiguous" % what)
+    have = x.numel() * x.element_size()
+    if have < nbytes:
+        raise BWError(BW_ERR_ARG, "%s holds %d bytes, the call needs %d" % (what, have, nbytes))
+    return C.c_void_p(x.data_ptr())
+
+
 def _stream(stream):
     if stream is None:
         import torch
@@ -267,10 +284,18 @@ class Plan:
 
     def execute(self, a, b, c, stream=None):
         """Device tensors: a (L_A limbs), b (L_B limbs) -> c (L_A+L_B limbs)."""
-        _check(lib().bw_execute(self._p, _ptr(a), _ptr(b), _ptr(c), _stream(stream)))
+        wb = self.limb_bits // 8
+        _check(lib().bw_execute(self._p, _dev(a, self.L_A * wb, "a"), _dev(b, self.L_B * wb, "b"),
+                                _dev(c, (self.L_A + self.L_B) * wb, "c"), _stream(stream)))
 
     def execute_crt(self, a, b, c0, c1, c2=None, stream=None):
-        _check(lib().bw_execute_crt(self._p, _ptr(a), _ptr(b), _ptr(c0), _ptr(c1), _ptr(c2), _stream(stream)))
+        """Early exit after Garner: N_exec coefficients (see bw_execute_crt)."""
+        wb = self.limb_bits // 8
+        n = self.N_exec
+        _check(lib().bw_execute_crt(self._p, _dev(a, self.L_A * wb, "a"), _dev(b, self.L_B * wb, "b"),
+                                    _dev(c0, 8 * n, "c0"), _dev(c1, 8 * n, "c1"),
+                                    _dev(c2, 4 * n, "c2") if (c2 is not None or self.limb_bits == 64) else None,
+                                    _stream(stream)))
 
     def execute_host(self, A, B):
         dt = _dt(self.limb_bits)
@@ -280,11 +305,18 @@ class Plan:
         _check(lib().bw_execute_host(self._p, _ptr(A), _ptr(B), _ptr(out)))
         return out
 
+    def _spec_bytes(self):
+        # residue words: 8 bytes under the 58-bit primes, 4 under the 31-bit ones
+        word = 8 if self.prime_set in (SET_U64_REF, SET_HYBRID, SET_U64_PAIRS) else 4
+        return self.k * self.N_exec * word
+
     def forward(self, raw, L, spec, stream=None):
-        _check(lib().bw_forward(self._p, _ptr(raw), L, _ptr(spec), _stream(stream)))
+        """Forward transforms of L raw limbs into spec (k planes of N_exec words)."""
+        _check(lib().bw_forward(self._p, _dev(raw, L * (self.limb_bits // 8), "raw"), L,
+                                _dev(spec, self._spec_bytes(), "spec"), _stream(stream)))
 
     def inverse(self, spec, stream=None):
-        _check(lib().bw_inverse(self._p, _ptr(spec), _stream(stream)))
+        _check(lib().bw_inverse(self._p, _dev(spec, self._spec_bytes(), "spec"), _stream(stream)))
 
     def spectrum_order(self):
         order = np.zeros(self.N_exec, dtype=np.uint64)
@@ -293,7 +325,9 @@ class Plan:
 
     def tune(self, a, b, stream=None, reps=3):
         """Autotune the pass split / tile cap on device operands a, b."""
-        _check(lib().bw_plan_tune(self._p, _ptr(a), _ptr(b), _stream(stream), reps))
+        wb = self.limb_bits // 8
+        _check(lib().bw_plan_tune(self._p, _dev(a, self.L_A * wb, "a"), _dev(b, self.L_B * wb, "b"),
+                                  _stream(stream), reps))
         info = PlanInfo()
         _check(lib().bw_plan_get_info(self._p, C.byref(info)))
         self.info = info
@@ -337,28 +371,34 @@ def multiply(A, B, limb_bits=None):
 # ---- stage-level entry points (device tensors) ------------------------------
 def zero_pad(src, dst, L, N, stream=None):
     wb = dst.element_size()
-    _check(lib().bw_zero_pad(_ptr(src), _ptr(dst), wb, L, N, _stream(stream)))
+    _check(lib().bw_zero_pad(_dev(src, 4 * L, "src"), _dev(dst, wb * N, "dst"), wb, L, N, _stream(stream)))
 
 
 def zero_pad_u64(src, dst, L, N, p, stream=None):
-    _check(lib().bw_zero_pad_u64(_ptr(src), _ptr(dst), L, N, p, _stream(stream)))
+    _check(lib().bw_zero_pad_u64(_dev(src, 8 * L, "src"), _dev(dst, 8 * N, "dst"), L, N, p, _stream(stream)))
 
 
 def pointwise(a, b, c, N, p, stream=None):
-    _check(lib().bw_pointwise(a.element_size(), _ptr(a), _ptr(b), _ptr(c), N, p, _stream(stream)))
+    wb = a.element_size()
+    _check(lib().bw_pointwise(wb, _dev(a, wb * N, "a"), _dev(b, wb * N, "b"), _dev(c, wb * N, "c"), N, p,
+                              _stream(stream)))
 
 
 def crt_combine(prime_set, r0, r1, r2, N, o0, o1, o2=None, stream=None):
-    _check(lib().bw_crt_combine(prime_set, _ptr(r0), _ptr(r1), _ptr(r2), N, _ptr(o0), _ptr(o1), _ptr(o2),
-                                _stream(stream)))
+    wb = 8 if prime_set in (SET_U64_REF, SET_HYBRID, SET_U64_PAIRS) else 4
+    _check(lib().bw_crt_combine(prime_set, _dev(r0, wb * N, "r0"), _dev(r1, wb * N, "r1"), _dev(r2, wb * N, "r2"),
+                                N, _dev(o0, 8 * N, "o0"), _dev(o1, 8 * N, "o1"),
+                                _dev(o2, 4 * N, "o2") if o2 is not None else None, _stream(stream)))
 
 
 def carry128(hi, lo, N, out, n_out, out_bits=32, stream=None):
-    _check(lib().bw_carry128(_ptr(hi), _ptr(lo), N, _ptr(out), n_out, out_bits, _stream(stream)))
+    _check(lib().bw_carry128(_dev(hi, 8 * N, "hi"), _dev(lo, 8 * N, "lo"), N,
+                             _dev(out, n_out * (out_bits // 8), "out"), n_out, out_bits, _stream(stream)))
 
 
 def carry160(lo, mid, hi, N, out, n_out, stream=None):
-    _check(lib().bw_carry160(_ptr(lo), _ptr(mid), _ptr(hi), N, _ptr(out), n_out, _stream(stream)))
+    _check(lib().bw_carry160(_dev(lo, 8 * N, "lo"), _dev(mid, 8 * N, "mid"), _dev(hi, 4 * N, "hi"), N,
+                             _dev(out, 8 * n_out, "out"), n_out, _stream(stream)))
 
 
 # ---- batched products ---------------------------------------------------------
@@ -439,13 +479,15 @@ class Batch:
 
     def execute(self, a, b, c, stream=None):
         """Packed device buffers (torch uint8 tensors) -> packed results in c."""
-        _check(lib().bw_batch_execute(self._b, _ptr(a), _ptr(b), _ptr(c), _stream(stream)))
+        _check(lib().bw_batch_execute(self._b, _dev(a, self.bytes_a, "a"), _dev(b, self.bytes_b, "b"),
+                                      _dev(c, self.bytes_c, "c"), _stream(stream)))
 
     def set_graph(self, on):
         _check(lib().bw_batch_set_graph(self._b, 1 if on else 0))
 
     def tune(self, a, b, stream=None, reps=3):
-        _check(lib().bw_batch_tune(self._b, _ptr(a), _ptr(b), _stream(stream), reps))
+        _check(lib().bw_batch_tune(self._b, _dev(a, self.bytes_a, "a"), _dev(b, self.bytes_b, "b"),
+                                   _stream(stream), reps))
 
     def status(self):
         _check(lib().bw_batch_status(self._b))
@@ -488,6 +530,11 @@ class Server:
     def submit(self, A, B, out, limb_bits=None):
         if limb_bits is None:
             limb_bits = 64 if A.dtype == np.uint64 else 32
+        dt = np.dtype(_dt(limb_bits))
+        for name, x, n in (("A", A, len(A)), ("B", B, len(B)), ("out", out, len(A) + len(B))):
+            if not isinstance(x, np.ndarray) or x.dtype != dt or not x.flags.c_contiguous or x.size < n:
+                raise BWError(BW_ERR_ARG, "%s must be a contiguous numpy %s array of at least %d limbs"
+                              % (name, dt.name, n))
         t = int(lib().bw_server_submit(self._s, limb_bits, _ptr(A), len(A), _ptr(B), len(B), _ptr(out)))
         if t < 0:
             _check(t)

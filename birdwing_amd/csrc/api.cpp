@@ -140,6 +140,7 @@ struct bw_plan {
     void* h_a = nullptr;
     void* h_b = nullptr;
     void* h_c = nullptr;
+    int* h_err = nullptr;           // pinned copy of counter[1] (host executions)
     hipStream_t own = nullptr;
     int schedule = BW_SCHED_PASS_MAJOR;
     int gshift = 0;
@@ -160,6 +161,7 @@ static void plan_free(bw_plan* p) {
     (void)hipFree(p->h_a);
     (void)hipFree(p->h_b);
     (void)hipFree(p->h_c);
+    if (p->h_err) (void)hipHostFree(p->h_err);
     if (p->own) (void)hipStreamDestroy(p->own);
     for (auto& set : p->evpool)
         for (auto e : set) (void)hipEventDestroy(e);
@@ -403,8 +405,26 @@ static int ensure_host_buffers(bw_plan* p) {
     if ((rc = plan_alloc(&p->h_a, p->L_A * wb, p)) || (rc = plan_alloc(&p->h_b, p->L_B * wb, p)) ||
         (rc = plan_alloc(&p->h_c, (p->L_A + p->L_B) * wb, p)))
         return rc;
+    HIP_TRY(hipHostMalloc((void**)&p->h_err, sizeof(int), 0));
+    *p->h_err = 0;
     HIP_TRY(hipStreamCreate(&p->own));
     return BW_OK;
+}
+
+// Reads the plan's look-back error word (counter[1]) into *h_err on `st`.
+// The caller synchronizes `st` before looking at it.
+static int queue_error_readback(bw_plan* p, int* h_err, hipStream_t st) {
+    HIP_TRY(hipMemcpyAsync(h_err, p->counter + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    return BW_OK;
+}
+
+// A set error word fails the execution that set it and is cleared, so the
+// next execution starts clean.
+static int take_error_word(bw_plan* p, int err, hipStream_t st) {
+    if (!err) return BW_OK;
+    HIP_TRY(hipMemsetAsync(p->counter + 1, 0, sizeof(int), st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return fail(BW_ERR_STATE, "carry look-back wait exceeded its bound: the product is not valid");
 }
 
 extern "C" int bw_execute_host(bw_plan* p, const void* a, const void* b, void* c) {
@@ -416,8 +436,9 @@ extern "C" int bw_execute_host(bw_plan* p, const void* a, const void* b, void* c
     HIP_TRY(hipMemcpyAsync(p->h_b, b, p->L_B * wb, hipMemcpyHostToDevice, p->own));
     if ((rc = bw_execute(p, p->h_a, p->h_b, p->h_c, p->own))) return rc;
     HIP_TRY(hipMemcpyAsync(c, p->h_c, (p->L_A + p->L_B) * wb, hipMemcpyDeviceToHost, p->own));
+    if ((rc = queue_error_readback(p, p->h_err, p->own))) return rc;
     HIP_TRY(hipStreamSynchronize(p->own));
-    return BW_OK;
+    return take_error_word(p, *p->h_err, p->own);
 }
 
 extern "C" int bw_plan_tune(bw_plan* p, const void* d_a, const void* d_b, void* stream, int reps) {
@@ -517,9 +538,9 @@ extern "C" int bw_plan_status(bw_plan* p) {
     int rc = check_plan(p);
     if (rc) return rc;
     int err = 0;
+    HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(&err, p->counter + 1, sizeof(int), hipMemcpyDeviceToHost));
-    if (err) return fail(BW_ERR_STATE, "carry look-back wait exceeded its bound");
-    return BW_OK;
+    return take_error_word(p, err, nullptr);
 }
 
 extern "C" int bw_plan_set_timing(bw_plan* p, int enabled) {

@@ -456,9 +456,13 @@ extern "C" int bw_batch_execute(bw_batch* b, const void* d_a, const void* d_b, v
 extern "C" int bw_batch_status(bw_batch* b) {
     if (!b) return fail(BW_ERR_ARG, "batch is NULL");
     int err = 0;
-    HIP_TRY(hipMemcpy(&err, static_cast<unsigned char*>(b->ws) + b->L.off_err, sizeof(int), hipMemcpyDeviceToHost));
-    if (err) return fail(BW_ERR_STATE, "carry look-back wait exceeded its bound");
-    return BW_OK;
+    int* word = reinterpret_cast<int*>(static_cast<unsigned char*>(b->ws) + b->L.off_err);
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(&err, word, sizeof(int), hipMemcpyDeviceToHost));
+    if (!err) return BW_OK;
+    // reported once: the next execution starts with a clear word
+    HIP_TRY(hipMemset(word, 0, sizeof(int)));
+    return fail(BW_ERR_STATE, "carry look-back wait exceeded its bound: the batch's products are not valid");
 }
 
 namespace bw {

@@ -63,9 +63,11 @@ struct CarryParams {
 
 hipError_t bw_launch_garner_carry(const CarryParams& p, hipStream_t stream);
 int64_t bw_carry_tiles(int64_t n_out);
-// Clears the tile flags and the tile-id counter of a carry launch (a kernel,
-// so that the sequence can live in a hipGraph).
-hipError_t bw_launch_carry_reset(unsigned long long* flags, int64_t ntiles, int* counter, hipStream_t stream);
+// Clears the tile flags and the tile-id counter of a carry launch, and the
+// look-back error word when `err` is given (a kernel, so that the sequence can
+// live in a hipGraph).
+hipError_t bw_launch_carry_reset(unsigned long long* flags, int64_t ntiles, int* counter, hipStream_t stream,
+                                 int* err = nullptr);
 // Standalone Garner (coefficients out): u32 path -> (hi, lo); u64 path -> U160.
 hipError_t bw_launch_garner_only(int limb_bits, const void* r0, const void* r1, const void* r2, int64_t N,
                                  const GarnerDev& g, const F64Prime* fp,

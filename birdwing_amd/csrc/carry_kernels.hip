@@ -348,17 +348,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
 
 int64_t bw_carry_tiles(int64_t n_out) { return (n_out + BW_CARRY_TILE - 1) / BW_CARRY_TILE; }
 
-__global__ void carry_reset_kernel(unsigned long long* flags, int64_t ntiles, int* counter) {
+__global__ void carry_reset_kernel(unsigned long long* flags, int64_t ntiles, int* counter, int* err) {
     const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     for (int64_t i = i0; i < ntiles; i += (int64_t)gridDim.x * blockDim.x) flags[i] = 0ULL;
-    if (i0 == 0) *counter = 0;
+    if (i0 == 0) {
+        *counter = 0;
+        if (err) *err = 0;
+    }
 }
 
-hipError_t bw_launch_carry_reset(unsigned long long* flags, int64_t ntiles, int* counter, hipStream_t stream) {
+hipError_t bw_launch_carry_reset(unsigned long long* flags, int64_t ntiles, int* counter, hipStream_t stream,
+                                 int* err) {
     int64_t blocks = (ntiles + 255) / 256;
     if (blocks < 1) blocks = 1;
     if (blocks > 1024) blocks = 1024;
-    hipLaunchKernelGGL(carry_reset_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, flags, ntiles, counter);
+    hipLaunchKernelGGL(carry_reset_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, flags, ntiles, counter, err);
     return hipGetLastError();
 }
 

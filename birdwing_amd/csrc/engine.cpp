@@ -67,6 +67,17 @@ int tables_load(const char* path, int* set_id) {
     return BW_OK;
 }
 
+int find_tables(int set_id, DevTables** out) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    auto it = g_tabs.find(std::make_pair(dev, set_id));
+    if (it == g_tabs.end())
+        return fail(BW_ERR_STATE, "tables of prime set %d are not built on device %d (call bw_init first)", set_id, dev);
+    *out = it->second.get();
+    return BW_OK;
+}
+
 int get_tables(int set_id, DevTables** out) {
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));

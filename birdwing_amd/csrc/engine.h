@@ -24,6 +24,9 @@ struct DevTables {
 };
 // Tables of prime set `set_id` on the current device (built on first use).
 int get_tables(int set_id, DevTables** out);
+// The same tables only if they are already built on the current device (no
+// allocation, no copy); BW_ERR_STATE otherwise.
+int find_tables(int set_id, DevTables** out);
 // Whether a pass split can run: every pass within its kernel family (the
 // register-staged kernels up to 2^10 points -- the only ones reading 64-bit
 // digits --, the LDS kernels up to 2^13), and the last pass of a split of two

@@ -114,6 +114,7 @@ struct bw_server {
     std::vector<Flight*> slot_flight;
     std::vector<unsigned char*> pinned;       // descriptor staging per slot
     std::vector<size_t> pinned_cap;
+    std::vector<int*> errw;                   // pinned copy of each slot's look-back error word
     unsigned char* dmem = nullptr;
     Arena arena;
 
@@ -177,6 +178,8 @@ static int launch_flight(bw_server* s, Flight* f, const BatchLayout& L) {
         const size_t w = (size_t)r->bits / 8;
         HIP_TRY(hipMemcpyAsync(r->c, dC + L.offC[i], (r->la + r->lb) * w, hipMemcpyDeviceToHost, st));
     }
+    // the flight's carry look-back error word, read when its event completes
+    HIP_TRY(hipMemcpyAsync(s->errw[f->slot], ws + L.off_err, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipEventRecord(s->events[f->slot], st));
     return BW_OK;
 }
@@ -268,7 +271,10 @@ static void worker_main(bw_server* s) {
             if (!f) continue;
             const hipError_t q = hipEventQuery(s->events[i]);
             if (q == hipSuccess) {
-                complete(s, f, BW_OK);
+                // a look-back wait past its bound leaves the flight's products invalid
+                const bool bad = *s->errw[i] != 0;
+                if (bad) s->worker_error = "carry look-back wait exceeded its bound";
+                complete(s, f, bad ? BW_ERR_STATE : BW_OK);
                 s->cv_done.notify_all();
             } else if (q != hipErrorNotReady) {
                 s->worker_error = hipGetErrorString(q);
@@ -332,6 +338,7 @@ extern "C" int bw_server_create(bw_server** out, const bw_server_config* cfg) {
     s->slot_flight.assign(s->nslots, nullptr);
     s->pinned.assign(s->nslots, nullptr);
     s->pinned_cap.assign(s->nslots, 0);
+    s->errw.assign(s->nslots, nullptr);
     int rc = BW_OK;
     hipError_t e = hipGetDevice(&s->device);
     if (e == hipSuccess) e = hipMalloc((void**)&s->dmem, c.device_bytes);
@@ -340,6 +347,8 @@ extern "C" int bw_server_create(bw_server** out, const bw_server_config* cfg) {
     for (int i = 0; i < s->nslots && !rc; i++) {
         e = hipStreamCreateWithFlags(&s->streams[i], hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&s->events[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&s->errw[i], sizeof(int), 0);
+        if (e == hipSuccess) *s->errw[i] = 0;
         if (e != hipSuccess) rc = fail(BW_ERR_HIP, "server streams: %s", hipGetErrorString(e));
     }
     // tables of every prime set a request can need, before the first request
@@ -446,6 +455,7 @@ extern "C" void bw_server_destroy(bw_server* s) {
         if (s->streams[i]) (void)hipStreamDestroy(s->streams[i]);
         if (s->events[i]) (void)hipEventDestroy(s->events[i]);
         if (s->pinned[i]) (void)hipHostFree(s->pinned[i]);
+        if (s->errw[i]) (void)hipHostFree(s->errw[i]);
     }
     (void)hipFree(s->dmem);
     delete s;

@@ -148,10 +148,15 @@ int bw_plan_set_schedule(bw_plan* plan, int schedule);
  * keeps the fastest, and records it in the process-wide table that later
  * plans of the same limb width and size start from. */
 int bw_plan_tune(bw_plan* plan, const void* d_a, const void* d_b, void* stream, int reps);
-/* Synchronizes and reports BW_ERR_STATE if a carry look-back wait of this plan
- * ever exceeded its bound (a failure that would otherwise be a hang). */
+/* Synchronizes the device and reports BW_ERR_STATE if a carry look-back wait
+ * of this plan exceeded its bound (a failure that would otherwise be a hang;
+ * the products of the executions since the last check are then not valid).
+ * The error word is sticky across bw_execute calls until this call reads and
+ * clears it. */
 int bw_plan_status(bw_plan* plan);
-/* Host buffers; copies in, runs, copies out, synchronizes. */
+/* Host buffers; copies in, runs, copies out, synchronizes.  Returns
+ * BW_ERR_STATE (and clears the error word) if this execution's carry
+ * look-back wait exceeded its bound. */
 int bw_execute_host(bw_plan* plan, const void* h_a, const void* h_b, void* h_c);
 /* Per-launch timing (hipEvents recorded on the execution's stream between
  * launches).  bw_plan_set_timing(plan, 1) starts a recording; bw_plan_timing
@@ -236,7 +241,9 @@ int bw_batch_set_graph(bw_batch* batch, int enabled);
 int bw_batch_tune(bw_batch* batch, const void* d_a, const void* d_b, void* stream, int reps);
 int bw_tune_count(void);
 void bw_tune_clear(void);
-/* Synchronizes; BW_ERR_STATE if a carry look-back wait exceeded its bound. */
+/* Synchronizes the device; BW_ERR_STATE if a carry look-back wait exceeded its
+ * bound since the last check (sticky across executions until read here, then
+ * cleared). */
 int bw_batch_status(bw_batch* batch);
 
 /* ---- serving (no counterpart in the reference) -----------------------------
@@ -264,7 +271,9 @@ int bw_server_create(bw_server** server, const bw_server_config* config);
 int64_t bw_server_submit(bw_server* server, int limb_bits, const void* h_a, uint64_t L_A, const void* h_b,
                          uint64_t L_B, void* h_c);
 /* Blocks until the ticket completes (timeout_ms < 0: no limit); releases the
- * ticket and reports its submit-to-completion latency. */
+ * ticket and reports its submit-to-completion latency.  Returns the ticket's
+ * status: BW_ERR_STATE when the carry look-back wait of its batch exceeded its
+ * bound (the result is then not valid). */
 int bw_server_wait(bw_server* server, int64_t ticket, int timeout_ms, double* latency_ms);
 /* 1 when the ticket has completed, 0 while it is queued or running. */
 int bw_server_poll(bw_server* server, int64_t ticket);

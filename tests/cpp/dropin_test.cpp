@@ -52,7 +52,68 @@ static std::vector<T> random_limbs(size_t n, std::mt19937_64& rng) {
     return v;
 }
 
+#ifndef BW_EXTENDED_SIZES
+// The reference's size-policy expectations for each configuration
+// (tests/test_ntt_limits.cpp:140-311: configured limits, CRT bound, multiply
+// limits, accepted and rejected N, the benchmark's L_arg convention), checked
+// through the drop-in headers.  max_log: log2 of the largest N; crt_ok /
+// crt_bad: log2 of square sizes inside / beyond the CRT bound; larg_ok /
+// larg_bad: L_arg values (2^L limbs) inside / beyond the multiply limits.
+struct RefPolicy { int max_log, crt_ok, crt_bad, larg_ok, larg_bad; };
+#if defined(NATIVE_HOST_LIMBS)
+static constexpr RefPolicy kRef = {32, 31, 50, 30, 32};
+#elif LIMB_BITS == 64
+static constexpr RefPolicy kRef = {33, 32, 53, 32, 33};
+#else
+static constexpr RefPolicy kRef = {23, 23, 24, 22, 23};
+#endif
+
+static void reference_policy_checks() {
+    const size_t one = 1;
+    CHECK(max_supported_logN() == kRef.max_log);
+    CHECK(max_supported_N() == one << kRef.max_log);
+    CHECK(max_supported_limb_count() == one << (kRef.max_log - 1));
+    std::string why;
+    CHECK(crt_coefficient_bound_satisfied(one << 22, one << 22, &why));
+    CHECK(crt_coefficient_bound_satisfied(one << kRef.crt_ok, one << kRef.crt_ok, &why));
+    why.clear();
+    CHECK(!crt_coefficient_bound_satisfied(one << kRef.crt_bad, one << kRef.crt_bad, &why));
+    CHECK(why.find("CRT modulus product") != std::string::npos);
+    CHECK(crt_coefficient_bound_satisfied(one << kRef.crt_bad, 1, &why));   // the short operand bounds the terms
+    CHECK(multiply_size_supported(one << 22, one << 22, &why));
+    CHECK(!multiply_size_supported(one << kRef.max_log, one << kRef.max_log, &why));   // needs N = 2^(max_log+1)
+    for (int logN = 1; logN <= kRef.max_log; logN++) CHECK(ntt_size_supported(one << logN, &why));
+    CHECK(ntt_size_supported(padded_ntt_size(one << (kRef.max_log - 1), one << (kRef.max_log - 1)), &why));
+    CHECK(!ntt_size_supported(0, &why));
+    CHECK(!ntt_size_supported(3, &why));
+    why.clear();
+    CHECK(!ntt_size_supported(one << (kRef.max_log + 1), &why));
+    CHECK(why.find(std::to_string(kRef.max_log + 1)) != std::string::npos);
+    CHECK(!ntt_size_supported(padded_ntt_size(one << kRef.max_log, one << kRef.max_log), &why));
+    CHECK(multiply_size_supported(one << kRef.larg_ok, one << kRef.larg_ok, &why));
+    CHECK(!multiply_size_supported(one << kRef.larg_bad, one << kRef.larg_bad, &why));
+}
+#else
+// BW_EXTENDED_SIZES: the library's automatic choice (32-bit limbs: three
+// primes up to N = 2^26, limb pairs under the 58-bit primes beyond).
+static void extended_policy_checks() {
+    const size_t one = 1;
+    std::string why;
+    CHECK(max_supported_logN() == (BW_CFG_LIMB_BITS == 32 ? 33 : 32));
+    CHECK(multiply_size_supported(one << 23, one << 23, &why));
+    CHECK(multiply_size_supported(one << 25, one << 25, &why));
+    CHECK(multiply_size_supported((one << 25) + 5, (one << 25) + 5, &why));
+    CHECK(ntt_size_supported(one << 24, &why) && ntt_size_supported(one << 30, &why));
+    CHECK(!multiply_size_supported(0, 5, &why));
+}
+#endif
+
 static void host_checks() {
+#ifndef BW_EXTENDED_SIZES
+    reference_policy_checks();
+#else
+    extended_policy_checks();
+#endif
     // per-prime limits of the configured moduli
     for (size_t i = 0; i < moduli.size(); i++) {
         const int v = max_logN_for_prime(moduli[i]);

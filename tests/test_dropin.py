@@ -22,6 +22,7 @@ CONFIGS = {
     "u32": ["-DLIMB_BITS=32"],
     "hybrid": ["-DLIMB_BITS=64"],
     "u64": ["-DLIMB_BITS=64", "-DNATIVE_HOST_LIMBS"],
+    "u32ext": ["-DLIMB_BITS=32", "-DBW_EXTENDED_SIZES"],
 }
 
 
