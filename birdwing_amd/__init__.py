@@ -99,6 +99,7 @@ SIGNATURES = {
     "bw_plan_set_timing": (_i, [_vp, _i]),
     "bw_plan_timing": (_i, [_vp, C.POINTER(C.c_float), _i, C.POINTER(_i)]),
     "bw_multiply_host": (_i, [_i, _vp, _u64, _vp, _u64, _vp, C.POINTER(C.c_double)]),
+    "bw_multiply_device_noalloc": (_i, [_i, _vp, _u64, _vp, _u64, _vp, _vp, _vp]),
     "bw_forward": (_i, [_vp, _vp, _u64, _vp, _vp]),
     "bw_inverse": (_i, [_vp, _vp, _vp]),
     "bw_spectrum_order": (_i, [_vp, C.POINTER(_u64)]),

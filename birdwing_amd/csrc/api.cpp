@@ -32,7 +32,10 @@
 
 using namespace bw;
 
-namespace bw { const char* last_error(); }
+namespace bw {
+const char* last_error();
+int noalloc_setup(int device);
+}
 extern "C" const char* bw_last_error(void) { return bw::last_error(); }
 extern "C" const char* bw_version(void) { return "birdwing-mi355x 0.2 (gfx950)"; }
 extern "C" int bw_abi_version(void) { return BW_ABI_VERSION; }
@@ -43,6 +46,8 @@ extern "C" int bw_init(int device) {
     int rc = get_tables(SET_U32_REF, &t);
     if (!rc) rc = get_tables(SET_U32_WIDE, &t);
     if (!rc) rc = get_tables(SET_D64, &t);
+    // the allocation-free entry point's scratch address on this device
+    if (!rc) rc = noalloc_setup(device);
     return rc;
 }
 

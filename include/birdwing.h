@@ -171,6 +171,27 @@ int bw_plan_timing(const bw_plan* plan, float* ms, int max, int* n);
 int bw_multiply_host(int limb_bits, const void* h_a, uint64_t L_A, const void* h_b, uint64_t L_B, void* h_c,
                      double* ms);
 
+/* ---- allocation-free product (no counterpart in the reference) -------------
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  One product
+ * of device buffers (d_a: L_A limbs, d_b: L_B limbs, d_c: L_A + L_B limbs of
+ * limb_bits 32 or 64), the same limbs as bw_execute, computed in a fixed
+ * device scratch array of about 10 MiB that each GPU holds until the process
+ * ends.  bw_init(device) must have been called for the current device: it
+ * sets up that device's scratch address and builds the tables the call uses.
+ * The call itself performs no hipMalloc, hipFree or other allocation.
+ * Limit: transforms of at most 2^18 64-bit digits, i.e.
+ * next_pow2(ceil(L_A/2) + ceil(L_B/2) - 1) <= 2^18 for 32-bit limbs and
+ * next_pow2(L_A + L_B - 1) <= 2^18 for 64-bit limbs (two 2^18-limb or two
+ * 2^17-limb operands); larger products return BW_ERR_UNSUPPORTED_SIZE.
+ * Every call on a GPU shares that GPU's scratch, which keeps the last call's
+ * contents: calls that overlap on one GPU (other streams or host threads)
+ * race; calls on one stream are ordered.  d_status (optional, a device int)
+ * is set to 0 by the call and to 1 if the carry look-back wait exceeded its
+ * bound (the result is then not valid).  Asynchronous on `stream`. */
+int bw_multiply_device_noalloc(int limb_bits, const void* d_a, uint64_t L_A, const void* d_b, uint64_t L_B,
+                               void* d_c, int* d_status, void* stream);
+
 /* ---- stage-level entry points ------------------------------------------- */
 /* Forward transforms of raw limbs (L of them, zero-padded to N_exec) under every
  * prime of the plan: d_spec gets k planes of N_exec words (u32 words on the
