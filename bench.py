@@ -1,10 +1,19 @@
 #!/usr/bin/env python3
 """bench.py -- MI355X big-integer multiply benchmark (one JSON line on stdout).
 
-Default workload: the plain single product of two 2^24-limb (32-bit limbs)
-operands, N = 2^25 per prime under the three ~30-bit primes
-{2013265921, 1811939329, 469762049}; inputs resident in HBM.  A step is one
-product; K steps are timed between device synchronizations.
+Default workload (`value`): the plain single product of two 2^24-limb
+operands of 32-bit limbs (uniform random, top limb nonzero), inputs resident
+in HBM.  It runs under the library's automatic method: five primes below 2^31
+on 64-bit digits (pairs of 32-bit limbs), N = 2^24 digits per prime, exact
+integer residues in FP64 arithmetic (`config.method`).  A step is one product;
+K steps are timed between device synchronizations.
+
+Beside it, on the same GPU in the same run (`legs`): the same product with
+64-bit limbs (five primes, N = 2^24), and the reference pipelines' configured
+three-prime methods at N = 2^25 -- the wide ~30-bit set on 32-bit limbs and
+the reference 58-bit set on 64-bit limbs -- each with per-kernel GB/s and the
+whole product against the staged byte model; the PCIe-inclusive plain time
+(host limb arrays in and out, `plain_host`); the batched and served workloads.
 
 Reported beside products/s:
   * p50/p99 latency of the timed products (hipEvents per step),
@@ -12,13 +21,18 @@ Reported beside products/s:
     staged one-pass byte model: each stage reads its logical inputs once and
     writes its outputs once) over its average launch duration, measured live
     with hipEvents on the launch stream during the timed region,
-  * the whole product's algorithmic bytes (136 B per coefficient on the 32-bit
-    path) over the time per product,
+  * the whole product's algorithmic bytes under the staged model of the
+    reference method (136 B per coefficient on 32-bit limbs, 248 B on 64-bit)
+    over the time per product,
   * bit-exactness of the GPU product against GMP mpz_mul,
-  * CPU baselines on this host: the oracle's NTT/Garner/carry pipeline (a port
-    of the reference's CPU path) and GMP mpz_mul, both on one bounded sample.
+  * CPU baselines on this host (core count and CPU model recorded): the
+    oracle's NTT/Garner/carry pipeline (a port of the reference's CPU path)
+    and GMP mpz_mul on one bounded sample of the same operands, and for the
+    batched and served workloads both as thread pools (one product per
+    thread).
 
 `--gpus N` with N > 1 exits: this build measures one GPU.
+`--valu-peak` runs the arithmetic-rate probes instead.
 """
 import argparse
 import json
@@ -180,7 +194,7 @@ def product_model_bytes(LA, LB, bits, k=3):
     return (int(LA) + int(LB)) * 2 * w + 8 * k * N * w + 2 * N * wc
 
 
-def bench_batch(n, steps, warmup, seed=4096, host_sample=128, tune=True, compare=True):
+def bench_batch(n, steps, warmup, seed=4096, host_sample=128, tune=True, compare=True, pool_seconds=8.0):
     """Batched workload: n products, limb counts per operand log-uniform in
     [2^8, 2^16] (seeded), uniform random 32-bit limbs (top limb nonzero);
     autotuned, hipGraph-captured; against the plain per-product path on the
@@ -273,6 +287,10 @@ def bench_batch(n, steps, warmup, seed=4096, host_sample=128, tune=True, compare
             same = same and bool(np.array_equal(pc[:m].cpu().numpy().view(np.uint32), got[i]))
     plain_s = (t3 - t2) / n
     del plans
+    if pool_seconds > 0:           # CPU thread pools over the same products
+        prods = [(32, pa[(b.off_a[i] >> 2):(b.off_a[i] >> 2) + LA[i]], pb[(b.off_b[i] >> 2):(b.off_b[i] >> 2) + LB[i]])
+                 for i in range(n)]
+        res["cpu_pool"] = cpu_pool(prods, pool_seconds, res["workload"])
     # host one-shot path (the reference's host_multiply_merge shape) on a sample
     idx = list(range(0, n, max(1, n // host_sample)))[:host_sample]
     t4 = time.perf_counter()
@@ -337,7 +355,8 @@ class _ServeMix:
         return v
 
 
-def bench_serve(seconds, seed=7, sat_seconds=10.0, plain_seconds=10.0):
+def bench_serve(seconds, seed=7, sat_seconds=10.0, plain_seconds=10.0, pool_seconds=8.0, sweep_mults=(2.0, 4.0, 8.0),
+                sweep_seconds=6.0):
     """Poisson request stream through the server at 80% of the plain path's
     sustained rate, a saturated run, and the plain per-request path (host
     one-shot multiply: plan, copies, execute, release per request)."""
@@ -425,8 +444,21 @@ def bench_serve(seconds, seed=7, sat_seconds=10.0, plain_seconds=10.0):
         srv.close()
         return len(lats) / elapsed, lats, st, checks
 
+    progress("serve: plain per-request path %.0f products/s" % plain_rate)
     served_rate, lat_s, st_s, checks = run(0.8 * plain_rate, seconds, check_every=50)
+    progress("serve: Poisson run done")
     sat_rate, lat_sat, st_sat, _ = run(None, sat_seconds, max_outstanding=256)
+    progress("serve: saturated run done")
+    # arrival rates above what the plain path sustains: continuous batching
+    # absorbs them (batch size grows with the queue)
+    sweep = []
+    for mult in sweep_mults:
+        r, l, st, _ = run(mult * plain_rate, sweep_seconds)
+        sweep.append({"arrival_rate": mult * plain_rate, "x_plain": mult, "products_per_s": r,
+                      "latency_ms": {"p50": float(np.percentile(l, 50)), "p99": float(np.percentile(l, 99))},
+                      "mean_batch": st["batched_products"] / max(1, st["batches"])})
+    if sweep_mults:
+        progress("serve: arrival-rate sweep done")
     # sampled results of the Poisson run equal the same products alone
     import pyoracle
     exact = True
@@ -437,6 +469,10 @@ def bench_serve(seconds, seed=7, sat_seconds=10.0, plain_seconds=10.0):
             g, _ = pyoracle.gmp_mul(A, B, bits)
             exact = exact and bool(np.array_equal(g, out))
     pct = lambda v, q: float(np.percentile(v, q)) if v else None
+    pool = None
+    if pool_seconds > 0:           # CPU thread pools over requests of the same mix
+        pmix = _ServeMix(seed + 2, ring_bytes=1 << 20)
+        pool = cpu_pool([pmix.next() for _ in range(4096)], pool_seconds, "serve_logu_2^6-2^22_80u32_20u64")
     return {
         "workload": "serve_poisson_logu_2^6-2^22_80u32_20u64",
         "seconds": seconds,
@@ -453,9 +489,211 @@ def bench_serve(seconds, seed=7, sat_seconds=10.0, plain_seconds=10.0):
         "plain_products_per_s": plain_rate,
         "plain_latency_ms": {"p50": pct(lat_plain, 50), "p99": pct(lat_plain, 99)},
         "saturated_vs_plain": sat_rate / plain_rate,
+        "sweep": sweep,
         "same_as_alone_sample": exact,
         "checked": len(checks),
+        "cpu_pool": pool,
     }
+
+
+def progress(msg):
+    """One line on stderr when a stage of the run ends (stdout keeps the JSON line)."""
+    print("bench: %s" % msg, file=sys.stderr, flush=True)
+
+
+def host_info():
+    """CPU model, logical CPUs and the CPUs this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cpus_available": avail}
+
+
+def pool_cores():
+    """Threads for the CPU thread pools: the CPUs this process may use, at
+    most 16 (a GPU box's CPU share per GPU)."""
+    return max(1, min(16, host_info()["cpus_available"] or 1))
+
+
+def cpu_pool(products, seconds, label):
+    """GMP mpz_mul and the oracle pipeline (one thread per product, no
+    threads inside a product) over `products` -- a list of (bits, A, B) --
+    on a thread pool of pool_cores() threads, each until about `seconds` of
+    wall time.  ctypes releases the GIL inside the C calls, so the threads
+    run in parallel.  Returns {"gmp": {...}, "oracle": {...}}."""
+    import concurrent.futures as cf
+    import threading
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    cores = pool_cores()
+    out = {}
+    fns = [("oracle", lambda b, x, y: pyoracle.multiply(x, y, b))]
+    if pyoracle.gmp_available():
+        fns.insert(0, ("gmp", lambda b, x, y: pyoracle.gmp_mul(x, y, b)))
+    pyoracle.set_prime_threads(False)
+    try:
+        for name, fn in fns:
+            lock = threading.Lock()
+            nxt = [0]
+            done = [0]
+            t0 = time.perf_counter()
+
+            def worker():
+                while time.perf_counter() - t0 < seconds:
+                    with lock:
+                        i = nxt[0]
+                        nxt[0] += 1
+                    b, x, y = products[i % len(products)]
+                    fn(b, x, y)
+                    with lock:
+                        done[0] += 1
+
+            with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+                for f in [ex.submit(worker) for _ in range(cores)]:
+                    f.result()
+            t = time.perf_counter() - t0
+            out[name] = {"value": done[0] / t, "unit": "products/s", "cores": cores,
+                         "kind": "library" if name == "gmp" else "port",
+                         "sample": "%d products of the %s workload in %.1f s, one product per thread" % (
+                             done[0], label, t)}
+    finally:
+        pyoracle.set_prime_threads(True)
+    out["host"] = host_info()
+    return out
+
+
+def bench_plain(A, B, bits, pset=-1, passes=None, schedule=-1, steps=20, warmup=3, tune=True, workload=None):
+    """One product shape, inputs resident in HBM: autotune (unless told not
+    to), W warm-up products, then K timed products between synchronizations
+    with per-launch hipEvents on the launch stream.  Returns (fields, plan
+    facts, the product limbs)."""
+    import re
+    import torch
+    import birdwing_amd as bw
+    LA, LB = len(A), len(B)
+    tdt = torch.int64 if bits == 64 else torch.int32
+    ndt = np.int64 if bits == 64 else np.int32
+    da = torch.from_numpy(A.view(ndt)).cuda()
+    db = torch.from_numpy(B.view(ndt)).cuda()
+    dc = torch.empty(LA + LB, dtype=tdt, device="cuda")
+    plan = bw.Plan(bits, LA, LB, pset, passes)
+    stream = torch.cuda.current_stream()
+    tuned = False
+    if not passes and tune:
+        plan.tune(da, db, stream, reps=3)      # pass split / tile cap / launch order, before any timing
+        tuned = True
+    if schedule >= 0:
+        plan.set_schedule(schedule)
+    sched = int(plan.info.schedule)
+    for _ in range(warmup):
+        plan.execute(da, db, dc, stream)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    plan.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for i in range(steps):
+        plan.execute(da, db, dc, stream)
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    plan.set_timing(False)
+    per_launch = plan.timing()
+    plan.status()
+    ms_step = (t1 - t0) * 1e3 / steps
+    lat = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(steps))
+    nl = launch_names(plan, sched)
+    names = [n for n, _ in nl]
+    lbytes = launch_bytes(plan, bits, nl)
+    agg = {}                                   # launches of one kernel type, across primes and operands
+    for n, t, by in zip(names, per_launch, lbytes):
+        key = re.sub(r"\[p\d\]|(?<=\))(A|B|AB)$", "", n)
+        a = agg.setdefault(key, [0.0, 0, 0])
+        a[0] += t
+        a[1] += by
+        a[2] += 1
+    dom_name = max(agg, key=lambda k: agg[k][0])
+    dom_ms, dom_bytes, dom_n = agg[dom_name]
+    dom_gbps = dom_bytes / (dom_ms * 1e-3) / 1e9
+    workload = workload or "plain_%dx%d_u%d" % (LA, LB, bits)
+    mbytes = model_bytes(plan, bits)
+    arith = "u64" if plan.prime_set in (bw.SET_U64_REF, bw.SET_HYBRID, bw.SET_U64_PAIRS) else "f64"
+    fields = {
+        "workload": workload,
+        "products_per_s": steps / (t1 - t0),
+        "ms_per_product": ms_step,
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "dtype": arith,
+        "config": {"workload": workload, "mode": "plain", "limbs_a": LA, "limbs_b": LB, "limb_bits": bits,
+                   "prime_set": plan.prime_set, "primes": plan.primes, "digit_bits": plan.digit_bits,
+                   "N": plan.N, "N_exec": plan.N_exec, "passes": plan.passes,
+                   "schedule": ["pass-major", "prime-major", "hybrid"][sched], "autotuned": tuned},
+        "kernels": [{"name": k, "launches": v[2], "ms_per_product": v[0], "alg_bytes": v[1],
+                     "alg_GBps": v[1] / (v[0] * 1e-3) / 1e9} for k, v in agg.items()],
+        "roofline": {"bound": "hbm", "kernel": dom_name, "launches_per_product": dom_n,
+                     "avg_launch_ms": dom_ms / dom_n, "alg_bytes_per_launch": dom_bytes / dom_n,
+                     "achieved": dom_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": dom_gbps / HBM_PEAK_GBPS, "traffic": load_traffic(workload, dom_name)},
+        "product_roofline": {"alg_bytes": mbytes, "achieved": mbytes / (ms_step * 1e-3) / 1e9,
+                             "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                             "frac": mbytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+                             "target_ms_at_half_peak": mbytes / (0.5 * HBM_PEAK_GBPS * 1e9) * 1e3},
+    }
+    out = dc.cpu().numpy().view(np.uint64 if bits == 64 else np.uint32).copy()
+    plan.close()
+    del da, db, dc
+    torch.cuda.empty_cache()
+    return fields, out
+
+
+METHOD_TEXT = {
+    "f64": "primes < 2^31 (%d of them) on %d-bit digits, N = 2^%d per prime, exact integer residues in FP64 "
+           "arithmetic",
+    "u64": "58-bit primes (%d of them) on %d-bit digits, N = 2^%d per prime, 64-bit integer arithmetic",
+}
+
+
+def method_of(f):
+    c = f["config"]
+    return METHOD_TEXT[f["dtype"]] % (len(c["primes"]), c["digit_bits"], int(c["N_exec"]).bit_length() - 1)
+
+
+def plain_host(A, B, bits, reps=5):
+    """The PCIe-inclusive plain path: host limb arrays in and out.
+    execute_host: a plan made beforehand, then copy in, run, copy out
+    (the reference's execute_ntt_multiply bucket, src/gpu_ntt.cu:468-473,
+    932-933); one-shot: plan creation, copies, run, release
+    (host_multiply_merge, src/host_multiply.cpp:27-86)."""
+    import birdwing_amd as bw
+    plan = bw.Plan(bits, len(A), len(B))
+    plan.execute_host(A, B)
+    t = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = plan.execute_host(A, B)
+        t.append((time.perf_counter() - t0) * 1e3)
+    plan.close()
+    t1 = []
+    for _ in range(max(2, reps // 2)):
+        t0 = time.perf_counter()
+        bw.multiply(A, B, bits)
+        t1.append((time.perf_counter() - t0) * 1e3)
+    return {"workload": "plain_%dx%d_u%d_host_buffers" % (len(A), len(B), bits),
+            "execute_host_ms": {"p50": float(np.percentile(t, 50)), "min": min(t)},
+            "one_shot_ms": {"p50": float(np.percentile(t1, 50)), "min": min(t1)},
+            "products_per_s": 1e3 / float(np.percentile(t, 50)),
+            "host_memory": "pageable numpy arrays"}, out
 
 
 def main():
@@ -465,12 +703,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--limbs", type=int, default=1 << 24, help="limbs per operand")
     ap.add_argument("--bits", type=int, default=32, choices=(32, 64))
+    ap.add_argument("--prime-set", type=int, default=-1, help="prime set of the main product (-1: automatic)")
     ap.add_argument("--passes", type=str, default="", help="pass split, e.g. 12,13")
     ap.add_argument("--schedule", type=int, default=-1,
                     help="launch order: 0 pass-major, 1 prime-major, 2 hybrid (default: the autotuner's choice)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-tune", action="store_true", help="keep the default pass split (no autotuning)")
     ap.add_argument("--no-oracle", action="store_true", help="GMP only as CPU baseline")
+    ap.add_argument("--no-legs", action="store_true", help="skip the 64-bit-limb and three-prime legs")
+    ap.add_argument("--leg-steps", type=int, default=10)
+    ap.add_argument("--no-host", action="store_true", help="skip the PCIe-inclusive plain path")
+    ap.add_argument("--pool-seconds", type=float, default=8.0,
+                    help="wall time of each CPU thread-pool baseline (0: skip)")
     ap.add_argument("--batch-n", type=int, default=4096, help="products in the batched workload (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-no-tune", action="store_true", help="batched workload with the default splits")
@@ -479,6 +723,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="Poisson serving run length (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device copies of MIB MiB (HBM counter calibration) and exit")
+    ap.add_argument("--valu-peak", action="store_true",
+                    help="only measure the VALU rates of FP64 FMA and 32-bit integer multiplies and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (--gpus 1); multi-GPU runs are not part of it",
@@ -497,115 +743,99 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "device copy", "bytes_read": n, "bytes_written": n, "copies": 3}))
         return
+    if args.valu_peak:
+        from birdwing_amd import probes
+        print(json.dumps(probes.valu_rates()))
+        return
 
     L = args.limbs
     bits = args.bits
     passes = [int(x) for x in args.passes.split(",")] if args.passes else None
     A = synthetic_limbs(L, 1, bits)
     B = synthetic_limbs(L, 2, bits)
-    tdt = torch.int64 if bits == 64 else torch.int32
-    ndt = np.int64 if bits == 64 else np.int32
-    da = torch.from_numpy(A.view(ndt)).cuda()
-    db = torch.from_numpy(B.view(ndt)).cuda()
-    dc = torch.empty(2 * L, dtype=tdt, device="cuda")
-    plan = bw.Plan(bits, L, L, bw.SET_DEFAULT, passes)
-    stream = torch.cuda.current_stream()
-    tuned = False
-    if not passes and not args.no_tune:
-        plan.tune(da, db, stream, reps=3)      # pass split / tile cap / launch order, before any timing
-        tuned = True
-    if args.schedule >= 0:
-        plan.set_schedule(args.schedule)
-    schedule = int(plan.info.schedule)
-
-    for _ in range(args.warmup):
-        plan.execute(da, db, dc, stream)
-    torch.cuda.synchronize()
-
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
-    plan.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev[0].record(stream)
-    for i in range(args.steps):
-        plan.execute(da, db, dc, stream)
-        ev[i + 1].record(stream)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    plan.set_timing(False)
-    per_launch = plan.timing()
-    plan.status()
-
-    elapsed = t1 - t0
-    ms_step = elapsed * 1e3 / args.steps
-    lat = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps))
-    p50 = float(np.percentile(lat, 50))
-    p99 = float(np.percentile(lat, 99))
-    nl = launch_names(plan, schedule)
-    names = [n for n, _ in nl]
-    lbytes = launch_bytes(plan, bits, nl)
-    # aggregate launches of one kernel type (across primes and operands)
-    import re
-    agg = {}
-    for n, t, by in zip(names, per_launch, lbytes):
-        key = re.sub(r"\[p\d\]|(?<=\))(A|B|AB)$", "", n)
-        a = agg.setdefault(key, [0.0, 0, 0])
-        a[0] += t
-        a[1] += by
-        a[2] += 1
-    dom_name = max(agg, key=lambda k: agg[k][0])
-    dom_ms, dom_bytes, dom_n = agg[dom_name]
-    dom_gbps = dom_bytes / (dom_ms * 1e-3) / 1e9
-    workload = "plain_%dx%d_u%d" % (L, L, bits)
-    traffic = load_traffic(workload, dom_name)
-    mbytes = model_bytes(plan, bits)
-
-    gpu_out = dc.cpu().numpy().view(np.uint64 if bits == 64 else np.uint32)
+    f, gpu_out = bench_plain(A, B, bits, args.prime_set, passes, args.schedule, args.steps, args.warmup,
+                             not args.no_tune)
+    progress("plain product timed (%.3f ms)" % f["ms_per_product"])
     cpu, exact = ({}, None) if args.no_cpu else cpu_baselines(A, B, bits, gpu_out, not args.no_oracle)
-
+    progress("CPU baselines done")
+    cfg = dict(f["config"])
+    cfg["method"] = method_of(f)
     line = {
         "metric": "products/s batched & served vs plain (bit-exact); p50/p99 latency; % HBM roofline",
-        "value": args.steps / elapsed,
+        "value": f["products_per_s"],
         "unit": "products/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": f["ms_per_product"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": f["dtype"],
         "data": "synthetic",
-        "config": {"workload": workload, "mode": "plain", "limbs_a": L, "limbs_b": L, "limb_bits": bits,
-                   "N": plan.N, "primes": plan.primes, "passes": plan.passes,
-                   "schedule": ["pass-major", "prime-major", "hybrid"][schedule], "autotuned": tuned},
-        "latency_ms": {"p50": p50, "p99": p99},
+        "config": cfg,
+        "latency_ms": f["latency_ms"],
         "bit_exact_vs_gmp": exact,
-        "kernels": [{"name": k, "launches": v[2], "ms_per_product": v[0], "alg_bytes": v[1],
-                     "alg_GBps": v[1] / (v[0] * 1e-3) / 1e9} for k, v in agg.items()],
-        "roofline": {"bound": "hbm", "kernel": dom_name, "launches_per_product": dom_n,
-                     "avg_launch_ms": dom_ms / dom_n, "alg_bytes_per_launch": dom_bytes / dom_n,
-                     "achieved": dom_gbps, "peak": HBM_PEAK_GBPS,
-                     "unit": "GB/s", "frac": dom_gbps / HBM_PEAK_GBPS,
-                     "traffic": traffic},
-        "product_roofline": {"alg_bytes": mbytes, "achieved": mbytes / (ms_step * 1e-3) / 1e9,
-                             "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                             "frac": mbytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBPS},
+        "kernels": f["kernels"],
+        "roofline": f["roofline"],
+        "product_roofline": f["product_roofline"],
     }
     if "oracle" in cpu:
-        line["cpu_baseline"] = cpu["oracle"]
+        line["cpu_baseline"] = dict(cpu["oracle"], **host_info())
     elif "gmp" in cpu:
-        line["cpu_baseline"] = cpu["gmp"]
+        line["cpu_baseline"] = dict(cpu["gmp"], **host_info())
     if "gmp" in cpu:
-        line["gmp_baseline"] = cpu["gmp"]
-    del plan, da, db, dc
+        line["gmp_baseline"] = dict(cpu["gmp"], **host_info())
+
+    if not args.no_host:
+        ph, hout = plain_host(A, B, bits)
+        ph["same_as_device"] = bool(np.array_equal(hout, gpu_out))
+        line["plain_host"] = ph
+        progress("host-buffer path timed")
+
+    if not args.no_legs and L == 1 << 24 and bits == 32 and args.prime_set < 0:
+        # the same operands under the reference pipeline's configured method
+        # (three ~30-bit primes at N = 2^25); its product must equal the main one
+        legs = {}
+        f3, out3 = bench_plain(A, B, 32, bw.SET_U32_WIDE, None, -1, args.leg_steps, 2, not args.no_tune,
+                               "plain_%dx%d_u32_3prime" % (L, L))
+        f3["method"] = method_of(f3)
+        f3["same_as_main"] = bool(np.array_equal(out3, gpu_out))
+        legs["three_prime_u32"] = f3
+        del out3
+        progress("three-prime leg (32-bit limbs) timed")
+        # 64-bit limbs: the automatic (five-prime) method and the reference 58-bit set
+        A64 = synthetic_limbs(L, 3, 64)
+        B64 = synthetic_limbs(L, 4, 64)
+        f64a, o64a = bench_plain(A64, B64, 64, -1, None, -1, args.leg_steps, 2, not args.no_tune)
+        f64a["method"] = method_of(f64a)
+        f64b, o64b = bench_plain(A64, B64, 64, bw.SET_U64_REF, None, -1, args.leg_steps, 2, not args.no_tune,
+                                 "plain_%dx%d_u64_3prime" % (L, L))
+        f64b["method"] = method_of(f64b)
+        same = bool(np.array_equal(o64a, o64b))
+        f64a["same_as_three_prime"] = f64b["same_as_five_prime"] = same
+        if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import pyoracle
+            if pyoracle.gmp_available():
+                g, _ = pyoracle.gmp_mul(A64, B64, 64)
+                f64a["bit_exact_vs_gmp"] = bool(np.array_equal(g, o64a))
+                f64b["bit_exact_vs_gmp"] = bool(np.array_equal(g, o64b))
+                del g
+        legs["u64"] = f64a
+        legs["three_prime_u64"] = f64b
+        progress("64-bit-limb legs timed")
+        del A64, B64, o64a, o64b
+        line["legs"] = legs
     torch.cuda.empty_cache()
     if args.batch_n > 0:
         line["batched"] = bench_batch(args.batch_n, args.batch_steps, 2, tune=not args.batch_no_tune,
-                                      compare=not args.batch_no_plain)
+                                      compare=not args.batch_no_plain, pool_seconds=args.pool_seconds)
         torch.cuda.empty_cache()
+        progress("batched workload done")
     if args.serve_seconds > 0:
-        line["served"] = bench_serve(args.serve_seconds)
+        line["served"] = bench_serve(args.serve_seconds, pool_seconds=args.pool_seconds)
+        progress("served workload done")
     print(json.dumps(line))
 
 

@@ -504,6 +504,12 @@ static uint64_t limb_at(const void* X, uint64_t i, int limb_bits) {
     return limb_bits == 64 ? ((const uint64_t*)X)[i] : ((const uint32_t*)X)[i];
 }
 
+/* Per-product parallelism: the primes of one product run on their own threads
+ * unless orc_set_prime_threads(0) asks for one thread per product (the
+ * thread-pool CPU baselines run one product per pool thread). */
+static int g_prime_threads = 1;
+void orc_set_prime_threads(int on) { g_prime_threads = on != 0; }
+
 int orc_multiply_crt(const void* A, uint64_t L_A, const void* B, uint64_t L_B, int limb_bits,
                      int set, uint64_t* c0, uint64_t* c1, uint32_t* c2) {
     if (!orc_multiply_size_supported(set, limb_bits, L_A, L_B)) return -1;
@@ -513,7 +519,7 @@ int orc_multiply_crt(const void* A, uint64_t L_A, const void* B, uint64_t L_B, i
     int k = orc_prime_set(set, P, R);
     uint64_t* res = (uint64_t*)malloc(sizeof(uint64_t) * N * (uint64_t)k);
     /* one prime per thread (the primes are independent until Garner) */
-    #pragma omp parallel for schedule(static, 1)
+    #pragma omp parallel for schedule(static, 1) if (g_prime_threads)
     for (int j = 0; j < k; j++) {
         uint64_t p = P[j], w = orc_root_for_N(p, R[j], logN);
         uint64_t* fa = res + (uint64_t)j * N;

@@ -103,6 +103,9 @@ void orc_carry160(const uint64_t* lo, const uint64_t* mid, const uint32_t* hi, u
 /* ---- whole product ---- */
 /* Prime set chosen by orc_choose_set (reference u32 set while N <= 2^23, wide set above). */
 int orc_choose_set(int limb_bits, uint64_t N);
+/* 1 (default): the primes of one product on their own OpenMP threads; 0: one
+ * thread per product (thread pools of products). */
+void orc_set_prime_threads(int on);
 /* out has L_A+L_B limbs.  Returns 0, or -1 if the size is unsupported. */
 int orc_multiply(const void* A, uint64_t L_A, const void* B, uint64_t L_B, void* out,
                  int limb_bits, int set);

@@ -70,6 +70,7 @@ def lib():
                                            C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
             "orc_schoolbook": (None, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int]),
             "orc_gmp_available": (C.c_int, []),
+            "orc_set_prime_threads": (None, [C.c_int]),
             "orc_gmp_mul": (C.c_double, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64, C.c_void_p, C.c_int]),
         }
         for name, (res, args) in sig.items():
@@ -258,6 +259,12 @@ def schoolbook(A, B, bits=32):
     out = np.zeros(len(A) + len(B), dtype=_dt(bits))
     lib().orc_schoolbook(_p(A), len(A), _p(B), len(B), _p(out), bits)
     return out
+
+
+def set_prime_threads(on):
+    """True: the primes of one multiply() run on their own threads (the
+    default); False: one thread per product (for thread pools of products)."""
+    lib().orc_set_prime_threads(1 if on else 0)
 
 
 def gmp_available():
