@@ -60,16 +60,22 @@ __device__ __forceinline__ void garner_d64(const uint32_t* r, const GarnerD64Dev
         double t = (double)(int32_t)r[j];
 #pragma unroll
         for (int i = 0; i < j; i++) t = f64_mulmod(t - kd[i], g.inv[j][i][0], g.inv[j][i][1], g.fp[j]);
-        kd[j] = f64_canon_d(t, g.fp[j]);
+        // t is in (-p_j, p_j): a residue word (canonical, or centered with
+        // |r| <= p/2 + 1) for j = 0, the output of a modular product for
+        // j > 0; one conditional add makes it canonical
+        kd[j] = t < 0.0 ? t + g.fp[j].p : t;
         k[j] = (uint32_t)kd[j];
     }
     uint32_t x[5] = {k[4], 0u, 0u, 0u, 0u};
 #pragma unroll
     for (int j = 3; j >= 0; j--) {
+        // the prime as a wave-uniform 32-bit scalar: each step is one
+        // 32 x 32 + 64-bit multiply-add (v_mad_u64_u32)
+        const uint32_t pj = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.p[j]);
         uint64_t c = k[j];
 #pragma unroll
         for (int i = 0; i < 5; i++) {
-            c += (uint64_t)x[i] * g.p[j];
+            c += (uint64_t)x[i] * pj;
             x[i] = (uint32_t)c;
             c >>= 32;
         }
@@ -89,9 +95,12 @@ struct CarryView {
     int64_t plane;
     int64_t N;
 };
+// WAVES: the least waves per SIMD the kernel is compiled for (a register
+// cap; each source at the most that compiles without spills).
 struct SrcGarnerU32 {
     using Word = uint32_t;
     static constexpr int NW = 3;
+    static constexpr int WAVES = 5;
     __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
         const bool in = n >= 0 && n < p.N;      // no branch: loads of later rows can issue early
         const int64_t m = in ? n : 0;
@@ -104,6 +113,7 @@ struct SrcGarnerU32 {
 struct SrcGarnerU64 {
     using Word = uint64_t;
     static constexpr int NW = 3;
+    static constexpr int WAVES = 4;
     __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
         const bool in = n >= 0 && n < p.N;
         const int64_t m = in ? n : 0;
@@ -115,6 +125,7 @@ struct SrcGarnerU64 {
 struct SrcGarnerD64 {
     using Word = uint64_t;
     static constexpr int NW = 3;
+    static constexpr int WAVES = 5;
     __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
         const bool in = n >= 0 && n < p.N;
         const uint32_t* a = reinterpret_cast<const uint32_t*>(p.a0) + (in ? n : 0);
@@ -128,6 +139,7 @@ struct SrcGarnerD64 {
 struct SrcHiLo128 {     // a0 = hi, a1 = lo
     using Word = uint32_t;
     static constexpr int NW = 4;
+    static constexpr int WAVES = 5;
     __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
         if (n < 0 || n >= p.N) { w[0] = w[1] = w[2] = w[3] = 0; return; }
         const uint64_t hi = reinterpret_cast<const uint64_t*>(p.a0)[n];
@@ -138,6 +150,7 @@ struct SrcHiLo128 {     // a0 = hi, a1 = lo
 struct SrcHiLo128L64 {  // a0 = hi, a1 = lo, 64-bit limbs
     using Word = uint64_t;
     static constexpr int NW = 2;
+    static constexpr int WAVES = 5;
     __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
         if (n < 0 || n >= p.N) { w[0] = w[1] = 0; return; }
         w[1] = reinterpret_cast<const uint64_t*>(p.a0)[n];
@@ -147,6 +160,7 @@ struct SrcHiLo128L64 {  // a0 = hi, a1 = lo, 64-bit limbs
 struct SrcU160 {        // a0 = lo, a1 = mid, a2 = hi32
     using Word = uint64_t;
     static constexpr int NW = 3;
+    static constexpr int WAVES = 5;
     __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
         if (n < 0 || n >= p.N) { w[0] = w[1] = w[2] = 0; return; }
         w[0] = reinterpret_cast<const uint64_t*>(p.a0)[n];
@@ -168,7 +182,7 @@ __device__ __forceinline__ Wd shift_up(Wd cur, Wd prev, int j, int lane) {
 #define FLAG_INC 2ULL
 
 template <class SRC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void garner_carry_kernel(CarryParams prm) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SRC::WAVES))) void garner_carry_kernel(CarryParams prm) {
     using Wd = typename SRC::Word;
     constexpr int NW = SRC::NW;
     constexpr int K = BW_CARRY_ROWS;
@@ -206,34 +220,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
 
     Wd tval[K];
     bool tg[K], tp[K];
-    Wd pw[NW];
-    Wd phi = 0;
-    // Row -1 supplies the lower neighbours of row 0 (its own lanes < NW-1 are never used).
-    {
-        Wd cw[NW];
-        SRC::load(view, prm, wbase - 64 + lane, cw);
-        Wd zero[NW];
-#pragma unroll
-        for (int j = 0; j < NW; j++) zero[j] = 0;
-        Wd lo = cw[0];
-        Wd hi = 0;
-#pragma unroll
-        for (int j = 1; j < NW; j++) {
-            const Wd v = shift_up(cw[j], zero[j], j, lane);
-            const Wd s = lo + v;
-            hi += (s < lo) ? 1 : 0;
-            lo = s;
-        }
-#pragma unroll
-        for (int j = 0; j < NW; j++) pw[j] = cw[j];
-        phi = hi;
-    }
-#pragma unroll
-    for (int i = 0; i < K; i++) {
-        Wd cw[NW];
-        SRC::load(view, prm, wbase + i * 64 + lane, cw);
-        Wd lo = cw[0];
-        Wd hi = 0;
+    // Limb n sums word j of coefficient n - j (j < NW), so row 0 of a wave
+    // needs the last NW - 1 coefficients of the wave before it (words j of
+    // its lanes 64 - j .. 63) and the high part of that row's last sum.
+    // Every wave hands those to its successor through LDS (slot wave + 1);
+    // wave 0 computes them for the tile before (slot 0).  Row 0 is therefore
+    // finished last, after a barrier, from its coefficient words kept in cw0.
+    __shared__ Wd s_tw[5][NW][NW];       // [slot][j][lane - (64 - NW)]: word j of the last lanes
+    __shared__ Wd s_thi[5];              // [slot]: high part of the last row's sum at lane 63
+    auto row_sums = [&](const Wd* cw, const Wd* pw, Wd& lo, Wd& hi) {
+        lo = cw[0];
+        hi = 0;
 #pragma unroll
         for (int j = 1; j < NW; j++) {
             const Wd v = shift_up(cw[j], pw[j], j, lane);
@@ -241,6 +238,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
             hi += (s < lo) ? 1 : 0;
             lo = s;
         }
+    };
+    auto publish_tail = [&](int slot, const Wd* cw, Wd hi) {
+        if (lane >= 64 - NW) {
+#pragma unroll
+            for (int j = 0; j < NW; j++) s_tw[slot][j][lane - (64 - NW)] = cw[j];
+        }
+        if (lane == 63) s_thi[slot] = hi;
+    };
+    Wd zero[NW];
+#pragma unroll
+    for (int j = 0; j < NW; j++) zero[j] = 0;
+    if (wave == 0) {
+        // the previous tile's last 64 limbs (zero before the product's start)
+        Wd cw[NW];
+        SRC::load(view, prm, wbase - 64 + lane, cw);
+        Wd lo, hi;
+        row_sums(cw, zero, lo, hi);
+        publish_tail(0, cw, hi);
+    }
+    Wd cw0[NW];
+    SRC::load(view, prm, wbase + lane, cw0);
+    Wd pw[NW];
+    Wd phi;
+    {
+        // row 0's high part at lane 63 does not depend on the row before it
+        Wd lo;
+        row_sums(cw0, zero, lo, phi);
+#pragma unroll
+        for (int j = 0; j < NW; j++) pw[j] = cw0[j];
+    }
+#pragma unroll
+    for (int i = 1; i < K; i++) {
+        Wd cw[NW];
+        SRC::load(view, prm, wbase + i * 64 + lane, cw);
+        Wd lo, hi;
+        row_sums(cw, pw, lo, hi);
         const Wd h1 = shift_up(hi, phi, 1, lane);
         const Wd t = lo + h1;
         tg[i] = t < lo;                        // carry out of the limb
@@ -249,6 +282,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void g
 #pragma unroll
         for (int j = 0; j < NW; j++) pw[j] = cw[j];
         phi = hi;
+    }
+    if (wave < 3) publish_tail(wave + 1, pw, phi);
+    __syncthreads();
+    {
+        // row 0 with the tail of the row before it
+        Wd prev[NW];
+        const int ti = lane >= 64 - NW ? lane - (64 - NW) : 0;     // in range for every lane
+#pragma unroll
+        for (int j = 0; j < NW; j++) prev[j] = s_tw[wave][j][ti];
+        Wd lo, hi;
+        row_sums(cw0, prev, lo, hi);           // lanes < j take word j of lane 64 - j + lane of prev
+        const Wd hs = shift_up(hi, hi, 1, lane);
+        const Wd h1 = lane == 0 ? s_thi[wave] : hs;
+        const Wd t = lo + h1;
+        tg[0] = t < lo;
+        tp[0] = !tg[0] && (t == ONES);
+        tval[0] = t;
     }
 
     // Wave summary over its K rows (cin = 0).
