@@ -194,7 +194,8 @@ def product_model_bytes(LA, LB, bits, k=3):
     return (int(LA) + int(LB)) * 2 * w + 8 * k * N * w + 2 * N * wc
 
 
-def bench_batch(n, steps, warmup, seed=4096, host_sample=128, tune=True, compare=True, pool_seconds=8.0):
+def bench_batch(n, steps, warmup, seed=4096, host_sample=128, tune=True, compare=True, pool_seconds=8.0,
+                graph_mode=1):
     """Batched workload: n products, limb counts per operand log-uniform in
     [2^8, 2^16] (seeded), uniform random 32-bit limbs (top limb nonzero);
     autotuned, hipGraph-captured; against the plain per-product path on the
@@ -223,22 +224,27 @@ def bench_batch(n, steps, warmup, seed=4096, host_sample=128, tune=True, compare
         with torch.cuda.stream(s):
             b.tune(da, db, stream=s, reps=2)
     t_tune = time.perf_counter() - t_tune
-    b.set_graph(True)
-    with torch.cuda.stream(s):
-        for _ in range(warmup):
-            b.execute(da, db, dc, stream=s)
-        s.synchronize()
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
-        t0 = time.perf_counter()
-        ev[0].record(s)
-        for i in range(steps):
-            b.execute(da, db, dc, stream=s)
-            ev[i + 1].record(s)
-        s.synchronize()
-        t1 = time.perf_counter()
-    b.status()
-    lat = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(steps))
-    batch_s = (t1 - t0) / steps
+    def timed(mode):
+        b.set_graph(mode)
+        with torch.cuda.stream(s):
+            for _ in range(warmup):
+                b.execute(da, db, dc, stream=s)
+            s.synchronize()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+            t0 = time.perf_counter()
+            ev[0].record(s)
+            for i in range(steps):
+                b.execute(da, db, dc, stream=s)
+                ev[i + 1].record(s)
+            s.synchronize()
+            t1 = time.perf_counter()
+        b.status()
+        return (t1 - t0) / steps, sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(steps))
+
+    # the other graph form first, for comparison; the reported one last
+    other = 1 if graph_mode == 2 else 2
+    other_s, _ = timed(other)
+    batch_s, lat = timed(graph_mode)
     out = dc.cpu().numpy()
     got = b.unpack(out)
     mbytes = sum(product_model_bytes(x, y, 32) for x, y in zip(LA, LB))
@@ -255,7 +261,9 @@ def bench_batch(n, steps, warmup, seed=4096, host_sample=128, tune=True, compare
         "peak": HBM_PEAK_GBPS,
         "unit": "GB/s",
         "frac": mbytes / batch_s / 1e9 / HBM_PEAK_GBPS,
-        "graph": True,
+        "graph": ["none", "one chain", "a branch per size group"][graph_mode],
+        "ms_per_batch_other_graph": {"graph": ["none", "one chain", "a branch per size group"][other],
+                                     "ms": other_s * 1e3},
         "launches": int(b.info.nlaunch),
         "autotuned": tune,
         "autotune_s": t_tune,

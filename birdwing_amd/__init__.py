@@ -484,7 +484,9 @@ class Batch:
                                       _dev(c, self.bytes_c, "c"), _stream(stream)))
 
     def set_graph(self, on):
-        _check(lib().bw_batch_set_graph(self._b, 1 if on else 0))
+        """False/0: plain launches; True/1: replayed hipGraph; 2: hipGraph with
+        one concurrent branch per size group."""
+        _check(lib().bw_batch_set_graph(self._b, int(on)))
 
     def tune(self, a, b, stream=None, reps=3):
         _check(lib().bw_batch_tune(self._b, _dev(a, self.bytes_a, "a"), _dev(b, self.bytes_b, "b"),

@@ -257,11 +257,24 @@ int batch_issue_group(const BatchLayout& L, const BatchGroup& g, void* ws, const
     return run_transforms(x, dA, dB, 0, 0, st, nullptr);
 }
 
-int batch_issue(const BatchLayout& L, void* ws, const void* dA, const void* dB, void* dC, hipStream_t st) {
+int batch_issue(const BatchLayout& L, void* ws, const void* dA, const void* dB, void* dC, hipStream_t st,
+                const BatchBranches* br) {
     unsigned char* d = static_cast<unsigned char*>(ws);
     int rc;
-    for (const auto& g : L.groups)
-        if ((rc = batch_issue_group(L, g, ws, dA, dB, st))) return rc;
+    if (br && !br->streams.empty() && L.groups.size() > 1) {
+        // groups are independent until Garner+carry: one branch each
+        HIP_TRY(hipEventRecord(br->fork, st));
+        for (size_t i = 0; i < L.groups.size(); i++) {
+            hipStream_t s = br->streams[i % br->streams.size()];
+            HIP_TRY(hipStreamWaitEvent(s, br->fork, 0));
+            if ((rc = batch_issue_group(L, L.groups[i], ws, dA, dB, s))) return rc;
+            HIP_TRY(hipEventRecord(br->joins[i], s));
+        }
+        for (size_t i = 0; i < L.groups.size(); i++) HIP_TRY(hipStreamWaitEvent(st, br->joins[i], 0));
+    } else {
+        for (const auto& g : L.groups)
+            if ((rc = batch_issue_group(L, g, ws, dA, dB, st))) return rc;
+    }
     for (const auto& c : L.carries) {
         DevTables* tabs;
         if ((rc = get_tables(c.set_id, &tabs))) return rc;
@@ -299,6 +312,8 @@ struct bw_batch {
     void* ws = nullptr;
     // hipGraph of the launch sequence, valid for (A, B, C, stream)
     bool use_graph = false;
+    bool branches = false;          // groups on concurrent streams (graph: one branch per group)
+    BatchBranches br;
     hipGraphExec_t exec = nullptr;
     hipGraph_t graph = nullptr;
     const void* gA = nullptr;
@@ -312,6 +327,32 @@ static void drop_graph(bw_batch* b) {
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
     b->graph = nullptr;
+}
+
+static void drop_branches(bw_batch* b) {
+    for (auto s : b->br.streams) (void)hipStreamDestroy(s);
+    for (auto e : b->br.joins) (void)hipEventDestroy(e);
+    if (b->br.fork) (void)hipEventDestroy(b->br.fork);
+    b->br = BatchBranches();
+}
+
+// Side streams (default priority) and events for concurrent groups, made once.
+static int make_branches(bw_batch* b) {
+    if (b->br.fork) return BW_OK;
+    const size_t ng = b->L.groups.size();
+    const size_t ns = ng < 8 ? ng : 8;
+    HIP_TRY(hipEventCreateWithFlags(&b->br.fork, hipEventDisableTiming));
+    for (size_t i = 0; i < ns; i++) {
+        hipStream_t s;
+        HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        b->br.streams.push_back(s);
+    }
+    for (size_t i = 0; i < ng; i++) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->br.joins.push_back(e);
+    }
+    return BW_OK;
 }
 
 static int upload_descriptors(bw_batch* b) {
@@ -364,6 +405,8 @@ extern "C" int bw_batch_footprint(int n, const int* limb_bits, const uint64_t* L
 extern "C" void bw_batch_destroy(bw_batch* b) {
     if (!b) return;
     drop_graph(b);
+    for (auto s : b->br.streams) (void)hipStreamSynchronize(s);
+    drop_branches(b);
     (void)hipFree(b->ws);
     delete b;
 }
@@ -417,8 +460,12 @@ extern "C" int bw_batch_group_info(const bw_batch* b, int gi, int* limb_bits, in
 
 extern "C" int bw_batch_set_graph(bw_batch* b, int enabled) {
     if (!b) return fail(BW_ERR_ARG, "batch is NULL");
+    if (enabled < 0 || enabled > 2) return fail(BW_ERR_ARG, "graph mode %d (0, 1 or 2)", enabled);
+    const bool branches = enabled == 2;
+    if (branches != b->branches || enabled == 0) drop_graph(b);
     b->use_graph = enabled != 0;
-    if (!b->use_graph) drop_graph(b);
+    b->branches = branches;
+    if (branches) return make_branches(b);
     return BW_OK;
 }
 
@@ -426,11 +473,12 @@ extern "C" int bw_batch_execute(bw_batch* b, const void* d_a, const void* d_b, v
     if (!b) return fail(BW_ERR_ARG, "batch is NULL");
     if (!d_a || !d_b || !d_c) return fail(BW_ERR_ARG, "NULL device buffer");
     hipStream_t st = (hipStream_t)stream;
+    const BatchBranches* br = b->branches ? &b->br : nullptr;
     if (!b->use_graph || st == nullptr) return batch_issue(b->L, b->ws, d_a, d_b, d_c, st);
     if (!b->exec || b->gA != d_a || b->gB != d_b || b->gC != d_c || b->gst != st) {
         drop_graph(b);
         HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        int rc = batch_issue(b->L, b->ws, d_a, d_b, d_c, st);
+        int rc = batch_issue(b->L, b->ws, d_a, d_b, d_c, st, br);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(st, &g);
         if (rc) {

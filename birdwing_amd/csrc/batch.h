@@ -76,8 +76,20 @@ void tune_candidates(int logN, std::vector<TuneChoice>& out);
 int batch_layout(int n, const int* bits, const uint64_t* LA, const uint64_t* LB, BatchLayout& L);
 // Host image of the descriptor block for a workspace at device address `ws`.
 void batch_descriptors(const BatchLayout& L, void* ws, std::vector<unsigned char>& img);
-// Issues the whole batch on `st`: transforms per group, then Garner+carry.
-int batch_issue(const BatchLayout& L, void* ws, const void* dA, const void* dB, void* dC, hipStream_t st);
+// Side streams and events for issuing the groups of a batch concurrently:
+// group i runs on streams[i % streams.size()] after a fork event recorded on
+// the main stream, and the main stream waits for every group's join event
+// before the Garner+carry launches.  Captured into a hipGraph this gives one
+// branch per group.
+struct BatchBranches {
+    std::vector<hipStream_t> streams;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> joins;   // one per group
+};
+// Issues the whole batch on `st`: transforms per group (on the branches when
+// given), then Garner+carry.
+int batch_issue(const BatchLayout& L, void* ws, const void* dA, const void* dB, void* dC, hipStream_t st,
+                const BatchBranches* br = nullptr);
 // Transform launches of one group only (autotuning).
 int batch_issue_group(const BatchLayout& L, const BatchGroup& g, void* ws, const void* dA, const void* dB,
                       hipStream_t st);

@@ -252,8 +252,12 @@ int bw_batch_group_info(const bw_batch* batch, int gi, int* limb_bits, int* logN
                         int* pass_log, int* npass, int* gshift);
 /* Runs the batch on `stream` with no host synchronization. */
 int bw_batch_execute(bw_batch* batch, const void* d_a, const void* d_b, void* d_c, void* stream);
-/* enabled: capture the launch sequence into a hipGraph on the first execute on
- * a non-null stream and replay it while (d_a, d_b, d_c, stream) stay the same. */
+/* enabled 1: capture the launch sequence into a hipGraph on the first execute
+ * on a non-null stream and replay it while (d_a, d_b, d_c, stream) stay the
+ * same; 2: the same with every size group's transforms on a branch of its own
+ * (the groups run concurrently until the Garner+carry launches join them);
+ * 0: plain launches in order.  On the default (null) stream the launches are
+ * always issued in order without a graph. */
 int bw_batch_set_graph(bw_batch* batch, int enabled);
 /* Autotuner: times candidate pass splits and tile caps of every group on the
  * given operands (reps timed runs each, synchronizing), keeps the fastest in

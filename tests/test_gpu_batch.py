@@ -124,6 +124,27 @@ def test_batch_graph_replays(gpu, oracle):
     _check_vs_gmp(oracle, 32, As, Bs, eager, sample=range(0, n, 7))
 
 
+@pytest.mark.parametrize("bits", [32, 64])
+def test_batch_graph_branches(gpu, oracle, bits):
+    """A graph with one concurrent branch per size group (mode 2), replayed
+    four times with eager work between, equals the eager in-order run; the
+    groups include products large enough for the long-stride inverse tiles."""
+    rng = np.random.default_rng(17 + bits)
+    n = 80
+    LA = _log_uniform(rng, n, 4, 18)
+    LB = _log_uniform(rng, n, 4, 18)
+    As, Bs = _operands(oracle, bits, LA, LB, 6)
+    b = gpu.Batch(bits, LA, LB)
+    pa, pb = b.pack(As, Bs)
+    print("branches: groups", [(g["logN"], g["members"]) for g in b.groups()], flush=True)
+    (eager,), _ = _run(b, pa, pb, graph=0)
+    reps, _ = _run(b, pa, pb, graph=2, reps=4)
+    for r in reps:
+        for i in range(n):
+            assert np.array_equal(r[i], eager[i]), i
+    _check_vs_gmp(oracle, bits, As, Bs, eager, sample=range(0, n, 9))
+
+
 def test_batch_autotune_keeps_results(gpu, oracle):
     rng = np.random.default_rng(13)
     n = 48
