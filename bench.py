@@ -32,7 +32,7 @@ Reported beside products/s:
     thread).
 
 `--gpus N` with N > 1 exits: this build measures one GPU.
-`--valu-peak` runs the arithmetic-rate probes instead.
+`--valu-peak` and `--mfma-dft16` run the arithmetic-rate probes instead.
 """
 import argparse
 import json
@@ -733,6 +733,8 @@ def main():
                     help="only run 3 device copies of MIB MiB (HBM counter calibration) and exit")
     ap.add_argument("--valu-peak", action="store_true",
                     help="only measure the VALU rates of FP64 FMA and 32-bit integer multiplies and exit")
+    ap.add_argument("--mfma-dft16", action="store_true",
+                    help="only time a 16-point DFT stage on int8 MFMA against the VALU codelet and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (--gpus 1); multi-GPU runs are not part of it",
@@ -751,9 +753,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "device copy", "bytes_read": n, "bytes_written": n, "copies": 3}))
         return
-    if args.valu_peak:
+    if args.valu_peak or args.mfma_dft16:
         from birdwing_amd import probes
-        print(json.dumps(probes.valu_rates()))
+        print(json.dumps(probes.valu_rates() if args.valu_peak else probes.mfma_dft16()))
         return
 
     L = args.limbs

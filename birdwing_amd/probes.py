@@ -4,7 +4,9 @@ libbirdwing_probe.so (birdwing_amd/probes/*.hip, built beside the product
 library) holds small kernels that measure what the product kernels' design
 rests on: the VALU rates of FP64 FMA against 32-bit integer multiplies, and
 of the three modular products (FP64, 32-bit Shoup, 64-bit Montgomery).
-bench.py --valu-peak prints valu_rates() as one JSON line.
+bench.py --valu-peak prints valu_rates() as one JSON line; bench.py
+--mfma-dft16 prints mfma_dft16(): one 16-point DFT stage mod p on int8 MFMA
+against the product's VALU codelet, on the same lines, timed and checked.
 """
 import ctypes as C
 import os
@@ -28,6 +30,9 @@ def lib():
         L = C.CDLL(PROBE_PATH)
         L.bwp_valu_rate.restype = C.c_double
         L.bwp_valu_rate.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.bwp_mfma_dft16.restype = C.c_int
+        L.bwp_mfma_dft16.argtypes = [C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_void_p, C.c_void_p,
+                                     C.c_int]
         _LIB = L
     return _LIB
 
@@ -53,3 +58,48 @@ def valu_rates(blocks=4096, iters=1 << 16):
         "f64_modmul_vs_mont_u64": rates["f64_modmul"] / rates["mont_u64_modmul"],
         "config": {"blocks": blocks, "threads": 256, "chains_per_lane": 8, "iters": iters},
     }
+
+
+def _dft16_rounds(x, p, w, rounds):
+    for _ in range(rounds):
+        x = [sum(x[j] * pow(w, j * k, p) for j in range(16)) % p for k in range(16)]
+    return x
+
+
+def mfma_dft16(lines=1 << 22, reps=5, rounds=(1, 8), nsample=32):
+    """Lines of 16 residues mod p = 2113929217 through the VALU radix-16
+    codelet and through int8 MFMA (see birdwing_amd/probes/mfma_probe.hip),
+    `rounds` DFT stages per line with a stage boundary (LDS round trip)
+    between; both store canonical residues in natural order.  The two outputs
+    are compared on the device (every element) and a sample of lines against
+    direct DFTs computed here.  One round is bound by HBM; eight rounds per
+    load compare the arithmetic."""
+    import numpy as np
+    import torch
+    torch.cuda.init()
+    out = {"probe": "mfma_dft16", "prime": 2113929217, "lines": lines, "runs": []}
+    for rnd in rounds:
+        res = (C.c_double * 5)()
+        hin = np.zeros(nsample * 16, dtype=np.int32)
+        hout = np.zeros(nsample * 16, dtype=np.uint32)
+        rc = lib().bwp_mfma_dft16(lines, reps, rnd, res, hin.ctypes.data_as(C.c_void_p),
+                                  hout.ctypes.data_as(C.c_void_p), nsample)
+        if rc != 0:
+            raise RuntimeError("mfma_dft16 probe failed")
+        p, w = 2113929217, int(res[4])
+        ok = 0
+        for i in range(nsample):
+            x = [int(v) % p for v in hin[16 * i:16 * i + 16]]
+            ok += int(_dft16_rounds(x, p, w, rnd) == [int(v) for v in hout[16 * i:16 * i + 16]])
+        tv, tm, n = res[0], res[1], res[2]
+        out["runs"].append({
+            "rounds": rnd,
+            "valu_codelet": {"seconds": tv, "dft16_per_s": rnd * n / tv, "elements_per_s": 16 * rnd * n / tv},
+            "mfma_i8": {"seconds": tm, "dft16_per_s": rnd * n / tm, "elements_per_s": 16 * rnd * n / tm},
+            "mfma_vs_valu": tv / tm,
+            "outputs_differing": int(res[3]),
+            "sample_lines_matching_direct_dft": "%d/%d" % (ok, nsample),
+        })
+    out["config"] = {"reps": reps, "lines_per_mfma_block": 16, "mfma": "v_mfma_i32_16x16x64_i8",
+                     "mfmas_per_block_and_round": 7}
+    return out
