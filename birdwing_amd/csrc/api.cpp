@@ -190,7 +190,9 @@ extern "C" int bw_plan_create_ex(bw_plan** out, int limb_bits, uint64_t L_A, uin
     const PrimeSet* s = prime_set(set_id);
     if (!s) return fail(BW_ERR_ARG, "unknown prime set %d", set_id);
     const bool d64 = s->digit_bits == 64;      // 64-bit digits (limb pairs or 64-bit limbs)
-    if (d64 ? !(s->k == 5 || s->id == SET_U64_PAIRS) : (s->limb_bits != limb_bits || s->k != 3))
+    const bool hybrid = s->id == SET_HYBRID;  // 32-bit limbs under two 58-bit primes
+    if (hybrid ? limb_bits != 32
+               : d64 ? !(s->k == 5 || s->id == SET_U64_PAIRS) : (s->limb_bits != limb_bits || s->k != 3))
         return fail(BW_ERR_ARG, "prime set %d does not serve %d-bit limbs", set_id, limb_bits);
     std::string why;
     if (!multiply_size_supported(*s, limb_bits, L_A, L_B, &why))
@@ -202,7 +204,7 @@ extern "C" int bw_plan_create_ex(bw_plan** out, int limb_bits, uint64_t L_A, uin
     bw_plan* p = new bw_plan();
     p->limb_bits = limb_bits;
     p->arith_bits = s->limb_bits;
-    p->ingress = d64 ? 2 : 1;
+    p->ingress = d64 ? 2 : hybrid ? 3 : 1;
     p->in_A = (int64_t)(d64 ? L_A * (limb_bits / 32) : L_A);
     p->in_B = (int64_t)(d64 ? L_B * (limb_bits / 32) : L_B);
     p->n_words = d64 && limb_bits == 32 ? (L_A + L_B + 1) / 2 : L_A + L_B;
@@ -362,7 +364,9 @@ extern "C" int bw_execute(bw_plan* p, const void* d_a, const void* d_b, void* d_
     if ((rc = run_plan_transforms(p, d_a, d_b, st, mk))) return rc;
     CarryParams cp;
     std::memset(&cp, 0, sizeof(cp));
-    cp.src = p->set->k == 5 ? BW_SRC_GARNER_D64 : (p->arith_bits == 64 ? BW_SRC_GARNER_U64 : BW_SRC_GARNER_U32);
+    cp.src = p->set->k == 5 ? BW_SRC_GARNER_D64
+             : p->arith_bits == 64 ? (p->set->k == 2 ? BW_SRC_GARNER_U64X2 : BW_SRC_GARNER_U64)
+                                   : BW_SRC_GARNER_U32;
     cp.gd = p->tabs->gd;
     cp.half_last = p->half_last;
     cp.a0 = p->bufA;
@@ -397,7 +401,7 @@ extern "C" int bw_execute_crt(bw_plan* p, const void* d_a, const void* d_b, uint
     if ((rc = run_plan_transforms(p, d_a, d_b, st, mk))) return rc;
     const int64_t N = (int64_t)p->N_exec;
     const unsigned char* base = (const unsigned char*)p->bufA;
-    LAUNCH_TRY(bw_launch_garner_only(p->limb_bits, base, base + N * p->word, base + 2 * N * p->word, N,
+    LAUNCH_TRY(bw_launch_garner_only(p->arith_bits, base, base + N * p->word, base + 2 * N * p->word, N,
                                      p->tabs->garner, p->tabs->fp, d_c0, d_c1, d_c2, st));
     LAUNCH_TRY(mk.mark());
     return BW_OK;
@@ -687,8 +691,10 @@ extern "C" int bw_pointwise(int word_bytes, const void* a, const void* b, void* 
 extern "C" int bw_crt_combine(int set_id, const void* r0, const void* r1, const void* r2, uint64_t N, uint64_t* o0,
                               uint64_t* o1, uint32_t* o2, void* stream) {
     const PrimeSet* s = prime_set(set_id);
-    if (!s || s->k != 3) return fail(BW_ERR_ARG, "prime set %d has no 3-prime Garner", set_id);
-    if (!r0 || !r1 || !r2 || !o0 || !o1 || (s->limb_bits == 64 && !o2)) return fail(BW_ERR_ARG, "NULL buffer");
+    if (!s || !(s->k == 3 || s->id == SET_HYBRID) || s->digit_bits)
+        return fail(BW_ERR_ARG, "prime set %d has no Garner over its residue arrays", set_id);
+    if (!r0 || !r1 || (s->k == 3 && !r2) || !o0 || !o1 || (s->limb_bits == 64 && s->k == 3 && !o2))
+        return fail(BW_ERR_ARG, "NULL buffer");
     DevTables* t;
     int rc = get_tables(set_id, &t);
     if (rc) return rc;

@@ -23,6 +23,7 @@ enum {
     BW_SRC_U160 = 3,         // (lo, mid, hi32) arrays -> 64-bit limbs
     BW_SRC_HILO128_L64 = 4,  // (hi, lo) u64 arrays -> 64-bit limbs
     BW_SRC_GARNER_D64 = 5,   // 5 canonical u32 residue planes of 64-bit digits -> 64-bit limbs
+    BW_SRC_GARNER_U64X2 = 6, // 2 canonical u64 residue planes of 32-bit digits -> 32-bit limbs
 };
 
 // Batched carry: one launch covers many products; product j owns tiles
@@ -68,7 +69,9 @@ int64_t bw_carry_tiles(int64_t n_out);
 // live in a hipGraph).
 hipError_t bw_launch_carry_reset(unsigned long long* flags, int64_t ntiles, int* counter, hipStream_t stream,
                                  int* err = nullptr);
-// Standalone Garner (coefficients out): u32 path -> (hi, lo); u64 path -> U160.
-hipError_t bw_launch_garner_only(int limb_bits, const void* r0, const void* r1, const void* r2, int64_t N,
+// Standalone Garner (coefficients out): 32-bit-prime sets -> (hi, lo); the
+// three 58-bit primes -> U160 (lo, mid, hi32); the two 58-bit primes of the
+// hybrid set -> (hi, lo).  arith_bits: 32 or 64; g.k gives the prime count.
+hipError_t bw_launch_garner_only(int arith_bits, const void* r0, const void* r1, const void* r2, int64_t N,
                                  const GarnerDev& g, const F64Prime* fp,
                                  uint64_t* o0, uint64_t* o1, uint32_t* o2, hipStream_t stream);

@@ -46,6 +46,18 @@ __device__ __forceinline__ void garner_u64(uint64_t r0, uint64_t r1, uint64_t r2
     w2 = (uint64_t)((s1 >> 64) + (c >> 64));
 }
 
+// Two-prime Garner of the hybrid set (58-bit primes, p0 < p1, 32-bit digits):
+// x = r0 + p0 k1 with k1 = (r1 - r0) p0^-1 mod p1; x < p0 p1 < 2^117.
+__device__ __forceinline__ void garner_u64x2(uint64_t r0, uint64_t r1, const GarnerDev& g, uint64_t& lo,
+                                             uint64_t& hi) {
+    const uint64_t p1 = g.p[1];
+    const uint64_t k1 = u64_mulmod_shoup(u64_sub(r1, r0, p1), g.inv1_s[0], g.inv1_s[1], p1);
+    typedef unsigned __int128 u128;
+    const u128 x = (u128)g.p0 * k1 + r0;
+    lo = (uint64_t)x;
+    hi = (uint64_t)(x >> 64);
+}
+
 // Five-prime Garner for 64-bit digits (residues canonical or centered, |r| < p):
 // mixed-radix digits
 //   k_j = (...((r_j - k_0) p_0^-1 - k_1) p_1^-1 ... - k_{j-1}) p_{j-1}^-1 mod p_j
@@ -120,6 +132,20 @@ struct SrcGarnerU64 {
         const uint64_t* r = reinterpret_cast<const uint64_t*>(p.a0);
         garner_u64(r[m], r[p.plane + m], r[2 * p.plane + m], c.g, w[0], w[1], w[2]);
         if (!in) w[0] = w[1] = w[2] = 0;
+    }
+};
+struct SrcGarnerU64x2 {
+    using Word = uint32_t;
+    static constexpr int NW = 4;
+    static constexpr int WAVES = 5;
+    __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
+        const bool in = n >= 0 && n < p.N;
+        const int64_t m = in ? n : 0;
+        const uint64_t* r = reinterpret_cast<const uint64_t*>(p.a0);
+        uint64_t lo, hi;
+        garner_u64x2(r[m], r[p.plane + m], c.g, lo, hi);
+        w[0] = in ? (uint32_t)lo : 0u; w[1] = in ? (uint32_t)(lo >> 32) : 0u;
+        w[2] = in ? (uint32_t)hi : 0u; w[3] = in ? (uint32_t)(hi >> 32) : 0u;
     }
 };
 struct SrcGarnerD64 {
@@ -425,6 +451,7 @@ hipError_t bw_launch_garner_carry(const CarryParams& p, hipStream_t stream) {
     case BW_SRC_HILO128_L64: hipLaunchKernelGGL(garner_carry_kernel<SrcHiLo128L64>, grid, block, 0, stream, p); break;
     case BW_SRC_U160: hipLaunchKernelGGL(garner_carry_kernel<SrcU160>, grid, block, 0, stream, p); break;
     case BW_SRC_GARNER_D64: hipLaunchKernelGGL(garner_carry_kernel<SrcGarnerD64>, grid, block, 0, stream, p); break;
+    case BW_SRC_GARNER_U64X2: hipLaunchKernelGGL(garner_carry_kernel<SrcGarnerU64x2>, grid, block, 0, stream, p); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -454,12 +481,25 @@ __global__ void garner_u64_kernel(const uint64_t* r0, const uint64_t* r1, const 
     hi[n] = (uint32_t)w2;
 }
 
-hipError_t bw_launch_garner_only(int limb_bits, const void* r0, const void* r1, const void* r2, int64_t N,
+__global__ void garner_u64x2_kernel(const uint64_t* r0, const uint64_t* r1, int64_t N, GarnerDev g, uint64_t* hi,
+                                    uint64_t* lo) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    uint64_t l, h;
+    garner_u64x2(r0[n], r1[n], g, l, h);
+    lo[n] = l;
+    hi[n] = h;
+}
+
+hipError_t bw_launch_garner_only(int arith_bits, const void* r0, const void* r1, const void* r2, int64_t N,
                                  const GarnerDev& g, const F64Prime* fp,
                                  uint64_t* o0, uint64_t* o1, uint32_t* o2, hipStream_t stream) {
     if (N <= 0) return hipSuccess;
     dim3 grid((unsigned)((N + 255) / 256)), block(256);
-    if (limb_bits == 64)
+    if (arith_bits == 64 && g.k == 2)
+        hipLaunchKernelGGL(garner_u64x2_kernel, grid, block, 0, stream, (const uint64_t*)r0, (const uint64_t*)r1, N, g,
+                           o1, o0);
+    else if (arith_bits == 64)
         hipLaunchKernelGGL(garner_u64_kernel, grid, block, 0, stream, (const uint64_t*)r0, (const uint64_t*)r1,
                            (const uint64_t*)r2, N, g, o0, o1, o2);
     else

@@ -174,7 +174,7 @@ bool split_supported(const std::vector<int>& passes, int arith_bits, int ingress
         if (why) *why = "1 to 4 passes";
         return false;
     }
-    const int maxlog = ingress == 2 ? BW_REG2_MAX_LOG : BW_MAX_PASS_LOG;
+    const int maxlog = (ingress == 2 || ingress == 3) ? BW_REG2_MAX_LOG : BW_MAX_PASS_LOG;
     for (int i = 0; i < P; i++) {
         const int t = passes[i];
         const bool row = i == P - 1 && P >= 2 && arith_bits == 32 && t >= BW_ROW_MIN_LOG && t <= BW_ROW_MAX_LOG;

@@ -53,7 +53,8 @@ struct Xform {
     void* bufB = nullptr;
     const void* ninv = nullptr;     // per-prime N^-1 pairs for this N
     int schedule = 0;               // BW_SCHED_*
-    int ingress = 1;                // raw operands: 1 a limb per digit, 2 64-bit digits of 32-bit limb pairs
+    int ingress = 1;                // raw operands: 1 a limb per digit, 2 64-bit digits of 32-bit limb pairs,
+                                    // 3 32-bit limbs into 64-bit residue words
     const BwIngress* bdesc = nullptr;   // stacked ingress (B > 1), device memory
 };
 

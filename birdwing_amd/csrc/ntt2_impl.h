@@ -287,8 +287,9 @@ __device__ __forceinline__ typename AR::TW tw_lookup(int64_t e, const typename A
 // One workgroup = one tile of G lines (G a power of two, runtime) with
 // NT = G * R1 threads: thread i owns stage-0 group i and stage-1 groups
 // i, i + NT, ...  COLS (S > 1) and INGRESS (0: residue words in, 1: one raw
-// limb per digit, 2: 64-bit digits from pairs of 32-bit limbs) are compile-time so
-// the inner loops carry no branches.
+// limb per digit, 2: 64-bit digits from pairs of 32-bit limbs, 3: 32-bit limbs
+// into 64-bit residue words) are compile-time so the inner loops carry no
+// branches.
 template <class AR, int LOGT, int MODE, bool COLS, int INGRESS>
 __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int prime) {
     using V = typename AR::V;
@@ -360,7 +361,7 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
     // product (stacked products: line o reads product o, shift o * M); the
     // raw limb type: a 64-bit digit is two 32-bit limbs.  Column tiles hold
     // one product per tile, so raw, len and the shift are wave-uniform.
-    using RL = typename std::conditional<INGRESS == 2, uint32_t, typename AR::Raw>::type;
+    using RL = typename std::conditional<INGRESS == 2 || INGRESS == 3, uint32_t, typename AR::Raw>::type;
     auto ingress_src = [&](int which, const RL*& raw, int64_t& len, int64_t& shift) {
         raw = reinterpret_cast<const RL*>(prm.in[which]);
         len = prm.in_len[which];
@@ -374,11 +375,13 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
         }
     };
     // stage-0 input words of one operand (raw digits or residue words)
-    using IW = typename std::conditional<INGRESS == 2, uint64_t,
-                                         typename std::conditional<INGRESS == 1, typename AR::Raw, W>::type>::type;
+    using IW = typename std::conditional<
+        INGRESS == 2, uint64_t,
+        typename std::conditional<INGRESS == 1, typename AR::Raw,
+                                  typename std::conditional<INGRESS == 3, uint32_t, W>::type>::type>::type;
     auto digit_value = [&](IW w) -> V {
         if constexpr (INGRESS == 2) return AR::from_raw64(w, pc);
-        else return AR::from_raw(w, P);
+        else return AR::from_raw((typename AR::Raw)w, P);
     };
     // This thread's R0 stage-0 input words of one operand: digits base + m*step.
     // Raw operands are zero past their end.  Cases: every digit inside (loads
@@ -768,9 +771,18 @@ static PassKernel2 pick2_v(bool cols, int ingress) {
         return cols ? ntt2_kernel<AR, LOGT, MODE, true, 0> : ntt2_kernel<AR, LOGT, MODE, false, 0>;
     } else if constexpr (MODE == BW_MODE_FUSED) {
         if (cols) return nullptr;
+        if (ingress == 3) {
+            if constexpr (std::is_same<AR, ArU64x>::value) return ntt2_kernel<AR, LOGT, MODE, false, 3>;
+            return nullptr;
+        }
         if (ingress == 2) return ntt2_kernel<AR, LOGT, MODE, false, 2>;
         return ingress ? ntt2_kernel<AR, LOGT, MODE, false, 1> : ntt2_kernel<AR, LOGT, MODE, false, 0>;
     } else {
+        if (ingress == 3) {
+            if constexpr (std::is_same<AR, ArU64x>::value)
+                return cols ? ntt2_kernel<AR, LOGT, MODE, true, 3> : ntt2_kernel<AR, LOGT, MODE, false, 3>;
+            return nullptr;
+        }
         if (ingress == 2) return cols ? ntt2_kernel<AR, LOGT, MODE, true, 2> : ntt2_kernel<AR, LOGT, MODE, false, 2>;
         if (cols) return ingress ? ntt2_kernel<AR, LOGT, MODE, true, 1> : ntt2_kernel<AR, LOGT, MODE, true, 0>;
         return ingress ? ntt2_kernel<AR, LOGT, MODE, false, 1> : ntt2_kernel<AR, LOGT, MODE, false, 0>;

@@ -77,6 +77,7 @@ hipError_t bw_launch_pass(const PassLaunch& L, const PassParams& prm, hipStream_
                             : (size_t)L.G * lds_line(L.logT) * vbytes;
     if (L.G > bw_pass_max_G(L.logT, L.limb_bits, prm.S > 1) || (L.G & (L.G - 1))) return hipErrorInvalidValue;
     if (prm.bdesc && prm.ingress && !reg2) return hipErrorInvalidValue;   // stacked ingress: reg2 only
+    if (prm.ingress >= 2 && !reg2) return hipErrorInvalidValue;           // limb pairs / 32-bit limbs in 64-bit words
     // register-staged column tiles of 32-bit words address their T*S block
     // with 32-bit byte offsets from the tile's first element (8 bytes per
     // element for the raw 64-bit digits of the ingress pass)

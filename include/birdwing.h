@@ -17,7 +17,9 @@
  *   BW_SET_U32_REF  {0x2d000001, 0x23800001, 0x26800001}, N <= 2^23
  *   BW_SET_U32_WIDE {2013265921, 1811939329, 469762049},  N <= 2^26
  *   BW_SET_U64_REF  {0x400002600000001, 0x400004200000001, 0x400001100000001}
- *   BW_SET_HYBRID   first two u64 primes (u32 limbs; reserved)
+ *   BW_SET_HYBRID   the first two u64 primes on 32-bit limbs (the reference's hybrid
+ *                   pipeline, include/config.h:26-30, src/gpu_ntt.cu:40-43): 64-bit
+ *                   integer arithmetic, 128-bit two-prime Garner, N <= 2^33
  *   BW_SET_D64      {2113929217, 2013265921, 1811939329, 1711276033, 1107296257}:
  *                   64-bit digits (a pair of 32-bit limbs or one 64-bit limb),
  *                   five transforms of half the length, FP64 arithmetic,
@@ -208,8 +210,10 @@ int bw_zero_pad_u64(const uint64_t* d_src, uint64_t* d_dst, uint64_t L, uint64_t
 /* src/gpu_ntt.cu:216-238 -- c = a*b mod p elementwise (word_bytes 4 or 8). */
 int bw_pointwise(int word_bytes, const void* d_a, const void* d_b, void* d_c, uint64_t N, uint64_t p,
                  void* stream);
-/* src/crt_gpu.cu:278-302 -- Garner over k = 3 residue arrays of the prime set.
- * 32-bit sets: o0 = lo, o1 = hi (128-bit).  64-bit set: o0 = lo, o1 = mid, o2 = hi32. */
+/* src/crt_gpu.cu:278-302 -- Garner over the residue arrays of a prime set.
+ * 32-bit sets (3 arrays): o0 = lo, o1 = hi (128-bit).  64-bit set (3 arrays):
+ * o0 = lo, o1 = mid, o2 = hi32.  Hybrid set (2 arrays, r2 unused): o0 = lo,
+ * o1 = hi. */
 int bw_crt_combine(int prime_set, const void* d_r0, const void* d_r1, const void* d_r2, uint64_t N,
                    uint64_t* d_o0, uint64_t* d_o1, uint32_t* d_o2, void* stream);
 /* src/carry_prop.cu (3-pass scheme) replaced by a single-pass carry scan.

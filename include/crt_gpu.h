@@ -58,7 +58,7 @@ inline CrtState& crt_state() {
 }
 // prime set of libbirdwing_hip.so whose primes are exactly params.primes
 inline int crt_set_of(const CRTGarnerParams& g) {
-    for (int s = 0; s <= 2; s++) {
+    for (int s = 0; s <= 3; s++) {
         int k = 0;
         uint64_t P[3] = {0, 0, 0};
         if (bw_prime_set_info(s, &k, P, nullptr) != BW_OK || k != NUM_MODULI) continue;
@@ -83,8 +83,8 @@ inline void upload_residue_ptrs(const std::vector<TestDataTypeUint*>& c_dev) {
 inline void crt_combine_gpu(uint64_t* d_C_hi, uint64_t* d_C_lo, int N) {
     auto& st = bw_dropin::crt_state();
     const int set = bw_dropin::crt_set_of(st.params);
-    if (!st.have_params || set < 0 || NUM_MODULI != 3) {
-        std::fprintf(stderr, "[CRT] Garner parameters must be one of the library's 3-prime sets\n");
+    if (!st.have_params || set < 0 || (NUM_MODULI != 3 && set != BW_SET_HYBRID)) {
+        std::fprintf(stderr, "[CRT] Garner parameters must be one of the library's 3-prime sets or the hybrid pair\n");
         std::exit(1);
     }
     if (bw_crt_combine(set, st.res[0], st.res[1], st.res[2], (uint64_t)N, d_C_lo, d_C_hi, nullptr, nullptr) != BW_OK) {

@@ -110,7 +110,11 @@ inline NTTPrecomputed precompute_ntt(size_t N, PrecomputeTiming* timing_out = nu
     NTTPrecomputed pre;
     pre.N = N;
     while ((size_t(1) << pre.logN) < N) pre.logN++;
+#if LIMB_BITS == 64 && !defined(NATIVE_HOST_LIMBS)
+    pre.prime_set = BW_SET_HYBRID;        // the hybrid pipeline's two 58-bit primes (src/gpu_ntt.cu:40-43)
+#else
     pre.prime_set = bw_default_prime_set(BW_CFG_LIMB_BITS, N);
+#endif
     int k = 0;
     uint64_t P[3] = {0, 0, 0};
     if (bw_prime_set_info(pre.prime_set, &k, P, nullptr) != BW_OK) bw_dropin::die("precompute_ntt");

@@ -270,11 +270,10 @@ static void gpu_checks() {
         for (size_t i = 0; i < N; i++) CHECK(out[i] == (i < L ? (TestDataTypeUint)src[i] : 0));
         (void)hipFree(ds); (void)hipFree(dd);
     }
-    // Garner on the device, residues of random coefficients below p0 p1 p2
-#if NUM_MODULI == 3
+    // Garner on the device, residues of random coefficients below the moduli product
     {
         const int N = 4097;
-        std::vector<std::vector<TestDataTypeUint>> res(3, std::vector<TestDataTypeUint>(N));
+        std::vector<std::vector<TestDataTypeUint>> res(NUM_MODULI, std::vector<TestDataTypeUint>(N));
         std::vector<unsigned __int128> x128(N);
         std::vector<uint64_t> xl(N), xm(N);
         std::vector<uint32_t> xh(N);
@@ -295,10 +294,10 @@ static void gpu_checks() {
         }
 #else
         for (int i = 0; i < N; i++)
-            for (int j = 0; j < 3; j++) res[j][i] = (TestDataTypeUint)(rng() % moduli[j]);
+            for (int j = 0; j < NUM_MODULI; j++) res[j][i] = (TestDataTypeUint)(rng() % moduli[j]);
 #endif
-        std::vector<TestDataTypeUint*> dres(3);
-        for (int j = 0; j < 3; j++) {
+        std::vector<TestDataTypeUint*> dres(NUM_MODULI);
+        for (int j = 0; j < NUM_MODULI; j++) {
             hip_ok(hipMalloc((void**)&dres[j], N * sizeof(TestDataTypeUint)));
             hip_ok(hipMemcpy(dres[j], res[j].data(), N * sizeof(TestDataTypeUint), hipMemcpyHostToDevice));
         }
@@ -322,13 +321,14 @@ static void gpu_checks() {
         hip_ok(hipMemcpy(xm.data(), d1, N * 8, hipMemcpyDeviceToHost));
         for (int i = 0; i < N; i++) {
             const unsigned __int128 x = ((unsigned __int128)xm[i] << 64) | xl[i];
-            CHECK(x == crt_combine_many(moduli, {res[0][i], res[1][i], res[2][i]}));
+            std::vector<TestDataTypeUint> r;
+            for (int j = 0; j < NUM_MODULI; j++) r.push_back(res[j][i]);
+            CHECK(x == crt_combine_many(moduli, r));
         }
 #endif
         (void)hipFree(d0); (void)hipFree(d1);
         for (auto p : dres) (void)hipFree(p);
     }
-#endif
 }
 
 int main(int argc, char** argv) {

@@ -641,3 +641,85 @@ def test_row_fused_split_rules(gpu):
         gpu.Plan(32, 1 << 20, 1 << 20, gpu.SET_DEFAULT, [15, 5])       # not the last pass
     with pytest.raises(gpu.BWError):
         gpu.Plan(64, 1 << 16, 1 << 16, gpu.SET_U64_REF, [3, 14])       # 64-bit arithmetic
+
+
+# ---- the hybrid pipeline: 32-bit limbs under the two 58-bit primes -----------
+HYBRID = 3
+
+
+@pytest.mark.parametrize("logN", [1, 3, 6, 9, 12, 14, 17])
+def test_hybrid_forward_and_roundtrip(gpu, oracle, logN):
+    """BW_SET_HYBRID (src/gpu_ntt.cu:40-43): 32-bit limbs enter 64-bit residue
+    words; the forward NTT against the oracle's per prime, then the inverse
+    returns the zero-padded limbs."""
+    d = dev()
+    N = 1 << logN
+    L = max(1, N // 2)
+    plan = gpu.Plan(32, L, L, HYBRID)
+    assert plan.N_exec == N and plan.k == 2 and plan.prime_set == HYBRID
+    raw = oracle.uniform_limbs(L, 2000 + logN, 32)
+    spec = d.empty(plan.k * N, 64)
+    plan.forward(d.to_dev(raw), L, spec)
+    got = d.from_dev(spec).reshape(plan.k, N)
+    order = plan.spectrum_order()
+    P, R = oracle.prime_set(HYBRID)
+    assert list(P) == plan.primes
+    for j in range(plan.k):
+        x = np.zeros(N, np.uint64)
+        x[:L] = raw.astype(np.uint64) % P[j]
+        want = oracle.ntt_forward(x, P[j], oracle.root_for_N(P[j], R[j], logN))
+        nat = np.zeros(N, np.uint64)
+        nat[order.astype(np.int64)] = got[j].astype(np.uint64)
+        assert np.array_equal(nat, want), "prime %d" % j
+    plan.inverse(spec)
+    back = d.from_dev(spec).reshape(plan.k, N)
+    for j in range(plan.k):
+        want = np.zeros(N, np.uint64)
+        want[:L] = raw.astype(np.uint64) % P[j]
+        assert np.array_equal(back[j].astype(np.uint64), want)
+
+
+@pytest.mark.parametrize("LA,LB", [(1, 1), (1, 2), (3, 5), (100, 3), (1000, 1003), (4097, 4095), (5000, 17),
+                                   (1 << 15, 1 << 15), (70000, 1000), (1 << 20, (1 << 20) - 1)])
+def test_hybrid_products_vs_gmp(gpu, oracle, LA, LB):
+    A = oracle.uniform_limbs(LA, 300 + LA, 32)
+    B = oracle.uniform_limbs(LB, 301 + LB, 32)
+    plan = _product_check(gpu, oracle, A, B, 32, HYBRID)
+    assert plan.k == 2 and plan.info.digit_bits == 32
+
+
+def test_hybrid_all_ones_and_splits(gpu, oracle):
+    """All-ones operands (the largest coefficients) and several pass splits."""
+    L = 1 << 16
+    X = np.full(L, 0xFFFFFFFF, np.uint32)
+    _product_check(gpu, oracle, X, X, 32, HYBRID)
+    A = oracle.uniform_limbs(3001, 5, 32)
+    B = oracle.uniform_limbs(2999, 6, 32)
+    for passes in ([6, 7], [4, 9], [3, 4, 6], [10, 3]):
+        _product_check(gpu, oracle, A, B, 32, HYBRID, passes)
+
+
+@pytest.mark.parametrize("L", [4, 256, 1 << 15])
+def test_hybrid_execute_crt_matches_oracle(gpu, oracle, L):
+    """The hybrid pipeline's early exit: 128-bit coefficients (lo, hi) of its
+    two primes, equal to the oracle's Garner (src/crt_gpu.cu:157-203)."""
+    d = dev()
+    A = oracle.ref_pipeline_limbs(L, 700 + L)
+    B = oracle.ref_pipeline_limbs(L, 701 + L)
+    plan = gpu.Plan(32, L, L, HYBRID)
+    N = plan.N_exec
+    c0, c1 = d.empty(N, 64), d.empty(N, 64)
+    plan.execute_crt(d.to_dev(A), d.to_dev(B), c0, c1)
+    w0, w1, _ = oracle.multiply_crt(A, B, 32, HYBRID)
+    assert np.array_equal(d.from_dev(c0, N), w0[:N])
+    assert np.array_equal(d.from_dev(c1, N), w1[:N])
+    # and the standalone Garner over the same residues
+    rng = np.random.default_rng(3)
+    P, _ = oracle.prime_set(HYBRID)
+    x = [int(v) for v in rng.integers(0, 2**62, 64, dtype=np.uint64)]
+    x = [(a * (2**52) + 12345) % (P[0] * P[1]) for a in x]
+    r = [d.to_dev(np.array([v % p for v in x], dtype=np.uint64)) for p in P]
+    o0, o1 = d.empty(64, 64), d.empty(64, 64)
+    gpu.crt_combine(HYBRID, r[0], r[1], r[0], 64, o0, o1)
+    lo, hi = d.from_dev(o0, 64), d.from_dev(o1, 64)
+    assert [int(h) << 64 | int(l) for l, h in zip(lo, hi)] == x
