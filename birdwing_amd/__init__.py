@@ -26,7 +26,7 @@ __all__ = [
     "padded_ntt_size", "multiply_size_supported", "ntt_size_supported", "crt_bound_satisfied",
     "max_supported_logN", "max_supported_limb_count", "max_limb_count_by_crt", "default_prime_set",
     "prime_set_info", "crt_combine", "carry128", "carry160", "zero_pad", "zero_pad_u64", "pointwise",
-    "Batch", "batch_footprint", "tune_count", "tune_clear", "Server",
+    "Batch", "batch_footprint", "set_fused_limit", "tune_count", "tune_clear", "Server",
 ]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -116,6 +116,8 @@ SIGNATURES = {
     "bw_batch_offsets": (_i, [_vp, _pu64, _pu64, _pu64]),
     "bw_batch_group_info": (_i, [_vp, _i, _pi, _pi, _pi, _pi, _pi, _pi, _pi]),
     "bw_batch_execute": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "bw_batch_fused_info": (_i, [_vp, _pi, _pi]),
+    "bw_set_fused_limit": (_i, [_i]),
     "bw_batch_set_graph": (_i, [_vp, _i]),
     "bw_batch_tune": (_i, [_vp, _vp, _vp, _vp, _i]),
     "bw_tune_count": (_i, []),
@@ -138,6 +140,13 @@ def lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError("birdwing_amd: %s is missing; build it with `make -C %s`" % (LIB_PATH, HERE))
+        # torch ships its own HIP runtime under the same soname: load it first so
+        # that the library binds to it (one runtime per process, whichever of
+        # the two touches the GPU first)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
@@ -456,6 +465,12 @@ class Batch:
                         "passes": [pl[j] for j in range(npass.value)], "gshift": gsh.value})
         return out
 
+    def fused(self):
+        """(products run by the fused whole-product launch, their largest log2 N)."""
+        n, m = C.c_int(), C.c_int()
+        _check(lib().bw_batch_fused_info(self._b, C.byref(n), C.byref(m)))
+        return n.value, m.value
+
     def pack(self, As, Bs):
         """Lists of limb arrays -> packed host buffers (numpy uint8)."""
         pa = np.zeros(max(self.bytes_a, 1), dtype=np.uint8)
@@ -494,6 +509,16 @@ class Batch:
 
     def status(self):
         _check(lib().bw_batch_status(self._b))
+
+
+def set_fused_limit(max_log):
+    """Largest log2 transform length (0 .. 11) that batches and servers created
+    from now on run through the fused whole-product launch (0: none).  Returns
+    the previous limit."""
+    r = lib().bw_set_fused_limit(int(max_log))
+    if r < 0:
+        _check(r)
+    return r
 
 
 def batch_footprint(limb_bits, L_A, L_B):

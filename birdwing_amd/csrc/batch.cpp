@@ -2,6 +2,7 @@
 #include "batch.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 
@@ -33,6 +34,25 @@ bool tune_lookup(const TuneKey& k, TuneChoice* c) {
 void tune_store(const TuneKey& k, const TuneChoice& c) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune[k] = c;
+}
+
+static std::atomic<int> g_fused_limit{BW_FUSED_DEFAULT_LOG};
+
+int set_fused_limit(int max_log) { return g_fused_limit.exchange(max_log); }
+int fused_limit() { return g_fused_limit.load(); }
+
+// Workgroups of a fused launch: one per CU (its LDS and registers hold one).
+static int fused_grid(int nbundles) {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0, c = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c > 0)
+            cus = c;
+        else
+            return std::min(nbundles, 256);
+    }
+    return std::min(nbundles, cus);
 }
 
 // Slots of a one-pass group are padded to the rows tile so that every
@@ -72,6 +92,8 @@ int batch_layout(int n, const int* bits, const uint64_t* LA, const uint64_t* LB,
     L.offB.resize(n);
     L.offC.resize(n);
     std::map<std::pair<int, int>, int> gidx;   // (prime set, logN) -> group
+    const int flim = fused_limit();
+    std::vector<std::pair<int, int>> fused;    // (logN, product)
     for (int i = 0; i < n; i++) {
         const int b = bits[i];
         if (b != 32 && b != 64) return fail(BW_ERR_ARG, "product %d: limb_bits must be 32 or 64", i);
@@ -91,6 +113,10 @@ int batch_layout(int n, const int* bits, const uint64_t* LA, const uint64_t* LB,
         L.bytesC = align_up(L.bytesC + (LA[i] + LB[i]) * w, 256);
         const uint64_t Nx = N < 2 ? 2 : N;
         const int logN = valuation2(Nx);
+        if (set_id == BW_SET_D64 && logN <= flim) {
+            fused.push_back(std::make_pair(logN, i));
+            continue;
+        }
         auto key = std::make_pair(set_id, logN);
         auto it = gidx.find(key);
         if (it == gidx.end()) {
@@ -107,6 +133,21 @@ int batch_layout(int n, const int* bits, const uint64_t* LA, const uint64_t* LB,
         BatchGroup& g = L.groups[it->second];
         if (g.set_id != set_id) return fail(BW_ERR_STATE, "prime set differs inside a size group");
         g.members.push_back(i);
+    }
+    // fused products: largest transforms first (the job queue hands them out in order)
+    std::stable_sort(fused.begin(), fused.end(),
+                     [](const std::pair<int, int>& a, const std::pair<int, int>& b) { return a.first > b.first; });
+    for (const auto& f : fused) {
+        L.fused.push_back(f.second);
+        L.fused_max_log = std::max(L.fused_max_log, f.first);
+    }
+    // bundles: runs of one transform length, as many products as a workgroup holds
+    for (size_t f = 0; f < fused.size();) {
+        const int lg = fused[f].first;
+        size_t e = f;
+        while (e < fused.size() && fused[e].first == lg && (int)(e - f) < bw_fused_bundle_size(lg)) e++;
+        L.fbundles.push_back(BwProductBundle{(int32_t)f, (int32_t)(e - f)});
+        f = e;
     }
     // pass split and tile cap: tuned choice if known, else the default split
     for (auto& g : L.groups) {
@@ -160,6 +201,14 @@ int batch_layout(int n, const int* bits, const uint64_t* LA, const uint64_t* LB,
         c.off_tile_job = off;
         off = align_up(off + sizeof(int) * (size_t)c.ntiles, 64);
     }
+    L.off_fjobs = off;
+    off = align_up(off + sizeof(BwProductJob) * L.fused.size(), 64);
+    L.off_fbundles = off;
+    off = align_up(off + sizeof(BwProductBundle) * L.fbundles.size(), 64);
+    // the fused job counter starts at zero with every descriptor upload (a
+    // server reuses arena memory) and returns to zero after every launch
+    L.off_fcounter = off;
+    off = align_up(off + sizeof(int), 64);
     L.desc_bytes = off - L.off_desc;
     off = align_up(off, 256);
     for (auto& c : L.carries) {
@@ -201,6 +250,22 @@ void batch_descriptors(const BatchLayout& L, void* ws, std::vector<unsigned char
             ing[slot] = d;
         }
     }
+    BwProductJob* fj = reinterpret_cast<BwProductJob*>(at(L.off_fjobs));
+    for (size_t f = 0; f < L.fused.size(); f++) {
+        const int i = L.fused[f];
+        const PrimeSet& s = *prime_set(BW_SET_D64);
+        const uint64_t N = padded_size_for(s, L.bits[i], L.LA[i], L.LB[i]);
+        BwProductJob J;
+        J.a_off = (int64_t)L.offA[i];
+        J.b_off = (int64_t)L.offB[i];
+        J.c_off = (int64_t)L.offC[i];
+        J.la = (int64_t)L.LA[i];
+        J.lb = (int64_t)L.LB[i];
+        J.limb_bits = L.bits[i];
+        J.logN = valuation2(N < 2 ? 2 : N);
+        fj[f] = J;
+    }
+    std::memcpy(at(L.off_fbundles), L.fbundles.data(), sizeof(BwProductBundle) * L.fbundles.size());
     for (const auto& c : L.carries) {
         BwCarryJob* jobs = reinterpret_cast<BwCarryJob*>(at(c.off_jobs));
         int* tile_job = reinterpret_cast<int*>(at(c.off_tile_job));
@@ -257,13 +322,37 @@ int batch_issue_group(const BatchLayout& L, const BatchGroup& g, void* ws, const
     return run_transforms(x, dA, dB, 0, 0, st, nullptr);
 }
 
+static int issue_fused(const BatchLayout& L, void* ws, const void* dA, const void* dB, void* dC, hipStream_t st) {
+    if (L.fused.empty()) return BW_OK;
+    unsigned char* d = static_cast<unsigned char*>(ws);
+    DevTables* tabs;
+    int rc = get_tables(BW_SET_D64, &tabs);
+    if (rc) return rc;
+    FusedParams fp;
+    std::memset(&fp, 0, sizeof(fp));
+    fp.A = static_cast<const unsigned char*>(dA);
+    fp.B = static_cast<const unsigned char*>(dB);
+    fp.C = static_cast<unsigned char*>(dC);
+    fp.jobs = reinterpret_cast<const BwProductJob*>(d + L.off_fjobs);
+    fp.bundles = reinterpret_cast<const BwProductBundle*>(d + L.off_fbundles);
+    fp.nbundles = (int)L.fbundles.size();
+    fp.counter = reinterpret_cast<int*>(d + L.off_fcounter);
+    fp.consts = static_cast<const F64PrimeDev*>(tabs->t.consts);
+    fp.tw_stage = static_cast<const double2*>(tabs->t.tw_stage);
+    fp.gd = tabs->gd;
+    LAUNCH_TRY(bw_launch_product_fused(fp, fused_grid(fp.nbundles), st));
+    return BW_OK;
+}
+
 int batch_issue(const BatchLayout& L, void* ws, const void* dA, const void* dB, void* dC, hipStream_t st,
                 const BatchBranches* br) {
     unsigned char* d = static_cast<unsigned char*>(ws);
     int rc;
     if (br && !br->streams.empty() && L.groups.size() > 1) {
-        // groups are independent until Garner+carry: one branch each
+        // groups are independent until Garner+carry: one branch each, the
+        // whole small products on the main stream beside them
         HIP_TRY(hipEventRecord(br->fork, st));
+        if ((rc = issue_fused(L, ws, dA, dB, dC, st))) return rc;
         for (size_t i = 0; i < L.groups.size(); i++) {
             hipStream_t s = br->streams[i % br->streams.size()];
             HIP_TRY(hipStreamWaitEvent(s, br->fork, 0));
@@ -272,6 +361,7 @@ int batch_issue(const BatchLayout& L, void* ws, const void* dA, const void* dB, 
         }
         for (size_t i = 0; i < L.groups.size(); i++) HIP_TRY(hipStreamWaitEvent(st, br->joins[i], 0));
     } else {
+        if ((rc = issue_fused(L, ws, dA, dB, dC, st))) return rc;
         for (const auto& g : L.groups)
             if ((rc = batch_issue_group(L, g, ws, dA, dB, st))) return rc;
     }
@@ -369,9 +459,14 @@ extern "C" int bw_batch_create(bw_batch** out, int n, const int* limb_bits, cons
     bw_batch* b = new bw_batch();
     int rc = batch_layout(n, limb_bits, L_A, L_B, b->L);
     if (rc) { delete b; return rc; }
+    // tables are built here, never inside a captured launch sequence
     for (const auto& g : b->L.groups) {
         DevTables* t;
         if ((rc = get_tables(g.set_id, &t))) { delete b; return rc; }
+    }
+    if (!b->L.fused.empty()) {
+        DevTables* t;
+        if ((rc = get_tables(BW_SET_D64, &t))) { delete b; return rc; }
     }
     hipError_t e = hipMalloc(&b->ws, b->L.ws_bytes);
     if (e != hipSuccess) {
@@ -428,7 +523,7 @@ extern "C" int bw_batch_get_info(const bw_batch* b, bw_batch_info* info) {
         x.schedule = BW_SCHED_PASS_MAJOR;
         nl += count_transform_launches(x);
     }
-    info->nlaunch = nl + 2 * (int)b->L.carries.size();
+    info->nlaunch = nl + 2 * (int)b->L.carries.size() + (b->L.fused.empty() ? 0 : 1);
     info->use_graph = b->use_graph ? 1 : 0;
     return BW_OK;
 }
@@ -456,6 +551,19 @@ extern "C" int bw_batch_group_info(const bw_batch* b, int gi, int* limb_bits, in
         for (size_t j = 0; j < g.passes.size() && j < 4; j++) pass_log[j] = g.passes[j];
     if (gshift) *gshift = g.gshift;
     return BW_OK;
+}
+
+extern "C" int bw_batch_fused_info(const bw_batch* b, int* products, int* max_logN) {
+    if (!b) return fail(BW_ERR_ARG, "batch is NULL");
+    if (products) *products = (int)b->L.fused.size();
+    if (max_logN) *max_logN = b->L.fused_max_log;
+    return BW_OK;
+}
+
+extern "C" int bw_set_fused_limit(int max_log) {
+    if (max_log < 0 || max_log > BW_FUSED_MAX_LOG)
+        return fail(BW_ERR_ARG, "fused limit %d (0 .. %d)", max_log, BW_FUSED_MAX_LOG);
+    return set_fused_limit(max_log);
 }
 
 extern "C" int bw_batch_set_graph(bw_batch* b, int enabled) {

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "product_fused.h"
 
 namespace bw {
 
@@ -48,6 +49,11 @@ struct BatchLayout {
     uint64_t bytesA = 0, bytesB = 0, bytesC = 0;
     std::vector<BatchGroup> groups;
     std::vector<BatchCarry> carries;
+    // products run whole by the fused kernel (product_fused.h), largest first
+    std::vector<int> fused;
+    std::vector<BwProductBundle> fbundles;    // indices into fused
+    int fused_max_log = 0;
+    size_t off_fjobs = 0, off_fbundles = 0, off_fcounter = 0;
     size_t off_desc = 0, desc_bytes = 0;      // descriptor block (uploaded as one copy)
     size_t off_epoch = 0, off_err = 0;
     size_t ws_bytes = 0;                      // device workspace
@@ -71,6 +77,12 @@ void tune_store(const TuneKey& k, const TuneChoice& c);
 int log2_bucket(int64_t B);
 // Candidate (pass split, tile cap) pairs for transforms of 2^logN points.
 void tune_candidates(int logN, std::vector<TuneChoice>& out);
+
+// Largest transform (log2, 64-bit digits) that batches laid out from now on
+// run through the fused whole-product kernel; 0 runs every product through
+// stacked transforms.  Returns the previous value.
+int set_fused_limit(int max_log);
+int fused_limit();
 
 // Builds the layout (and validates every size).  Returns BW_OK or an error.
 int batch_layout(int n, const int* bits, const uint64_t* LA, const uint64_t* LB, BatchLayout& L);

@@ -230,10 +230,15 @@ int bw_carry160(const uint64_t* d_lo, const uint64_t* d_mid, const uint32_t* d_h
  * n products of uneven sizes; limb_bits[i] is 32 or 64 per product.  Operands
  * and results are packed: product i reads L_A[i] limbs at d_a + off_a[i] and
  * L_B[i] limbs at d_b + off_b[i] (byte offsets from bw_batch_offsets, 256-byte
- * aligned) and writes L_A[i] + L_B[i] limbs at d_c + off_c[i].  Products with
- * the same limb width and padded size share every launch; one Garner+carry
- * launch per prime set covers the whole batch.  The batch owns its device
- * workspace (bw_batch_info.workspace_bytes). */
+ * aligned) and writes L_A[i] + L_B[i] limbs at d_c + off_c[i].  Products
+ * whose five-prime transform has at most 2^bw_set_fused_limit 64-bit digits
+ * (default 2^8, at most 2^11) run whole in one persistent launch: a workgroup
+ * runs a bundle of up to 16 products of one length, transforms, pointwise
+ * product, Garner and carry in LDS, bundles taken largest first from a job
+ * counter.  The other products with the same
+ * limb width and padded size share every launch; one Garner+carry launch per
+ * prime set covers them.  The batch owns its device workspace
+ * (bw_batch_info.workspace_bytes). */
 typedef struct bw_batch bw_batch;
 typedef struct bw_batch_info {
     int n;                   /* products */
@@ -254,6 +259,14 @@ int bw_batch_offsets(const bw_batch* batch, uint64_t* off_a, uint64_t* off_b, ui
 /* Group gi: limb width, log2 N, products, stacked slots, pass split, tile cap. */
 int bw_batch_group_info(const bw_batch* batch, int gi, int* limb_bits, int* logN, int* members, int* slots,
                         int* pass_log, int* npass, int* gshift);
+/* Products the batch runs through the fused whole-product launch, and the
+ * largest log2 transform length among them (0 when there are none). */
+int bw_batch_fused_info(const bw_batch* batch, int* products, int* max_logN);
+/* Largest log2 transform length (64-bit digits, 0 .. 11) of the products that
+ * batches (and server batches) laid out from now on run through the fused
+ * whole-product launch; 0 sends every product through stacked transforms.
+ * Process-wide.  Returns the previous limit, or a negative status. */
+int bw_set_fused_limit(int max_log);
 /* Runs the batch on `stream` with no host synchronization. */
 int bw_batch_execute(bw_batch* batch, const void* d_a, const void* d_b, void* d_c, void* stream);
 /* enabled 1: capture the launch sequence into a hipGraph on the first execute

@@ -210,3 +210,106 @@ def test_batch_4096_uneven_checksums(gpu, oracle):
         bb = pb[b.off_b[i]:b.off_b[i] + 4 * int(LB[i])].view(np.uint32)
         want, _ = oracle.gmp_mul(a, bb, 32)
         assert np.array_equal(got[i], want), i
+
+
+# ---------------------------------------------------------------------------
+# Whole products in one workgroup (the fused launch of products up to 2^11
+# 64-bit digits, several of one length to a workgroup): every transform
+# length, the largest operands of each, all-ones operands, odd limb counts,
+# against GMP and against the same products run through stacked transforms.
+
+def _fused_specs(bits):
+    """(L_A, L_B) per transform length 2^L, L = 1 .. 11: the largest product of
+    that length (digits(A) + digits(B) - 1 = 2^L), an odd total of limbs, an
+    unbalanced one."""
+    per = 2 if bits == 32 else 1
+    specs = []
+    for L in range(1, 12):
+        n = 1 << L
+        da, db = n // 2 + 1, n // 2
+        specs.append((per * da, per * db))
+        if bits == 32:
+            specs.append((2 * da - 1, 2 * db))      # odd limb total: half of the top word
+        specs.append((per * (n - 1) + (1 if L > 1 else 0), 1))
+    return specs
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_fused_products_vs_gmp(gpu, oracle, bits):
+    specs = _fused_specs(bits)
+    LA = [s[0] for s in specs]
+    LB = [s[1] for s in specs]
+    As, Bs = _operands(oracle, bits, LA, LB, 21)
+    # all-ones operands for the largest product of every length: coefficients
+    # at the CRT bound's worst case for that length, carries through every limb
+    per_len = len(specs) // 11           # specs per transform length, the largest first
+    for i in range(0, len(specs), per_len):
+        As[i][:] = np.iinfo(As[i].dtype).max
+        Bs[i][:] = np.iinfo(Bs[i].dtype).max
+    old = gpu.set_fused_limit(11)
+    try:
+        b = gpu.Batch(bits, LA, LB)
+    finally:
+        gpu.set_fused_limit(old)
+    nf, mx = b.fused()
+    print("fused bits=%d products=%d fused=%d max_logN=%d groups=%s" % (bits, len(specs), nf, mx, b.groups()),
+          flush=True)
+    assert nf == len(specs) and mx == 11 and not b.groups()
+    pa, pb = b.pack(As, Bs)
+    (got,), _ = _run(b, pa, pb)
+    _check_vs_gmp(oracle, bits, As, Bs, got)
+    # a batch of fused products only, captured and replayed
+    reps, _ = _run(b, pa, pb, graph=True, reps=2)
+    for r in reps:
+        for i in range(len(specs)):
+            assert np.array_equal(r[i], got[i]), i
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_fused_equals_stacked(gpu, oracle, bits):
+    """The same mixed batch with the fused launch on and off: identical limbs
+    (the larger products run stacked in both)."""
+    rng = np.random.default_rng(31 + bits)
+    n = 160
+    LA = _log_uniform(rng, n, 0, 14)
+    LB = _log_uniform(rng, n, 0, 14)
+    As, Bs = _operands(oracle, bits, LA, LB, 8)
+    outs = []
+    for lim in (11, 0):
+        old = gpu.set_fused_limit(lim)
+        try:
+            b = gpu.Batch(bits, LA, LB)
+        finally:
+            gpu.set_fused_limit(old)
+        nf, _ = b.fused()
+        assert (nf > 0) == (lim > 0)
+        pa, pb = b.pack(As, Bs)
+        (got,), _ = _run(b, pa, pb)
+        outs.append(got)
+    for i in range(n):
+        assert np.array_equal(outs[0][i], outs[1][i]), (i, LA[i], LB[i])
+    _check_vs_gmp(oracle, bits, As, Bs, outs[0], sample=range(0, n, 5))
+
+
+def test_fused_graph_replays(gpu, oracle):
+    """The fused launch's job counter returns to zero after every launch:
+    graph replays and eager runs (both graph forms) give the same limbs."""
+    rng = np.random.default_rng(41)
+    n = 600
+    LA = _log_uniform(rng, n, 0, 12.5)
+    LB = _log_uniform(rng, n, 0, 12.5)
+    As, Bs = _operands(oracle, 32, LA, LB, 9)
+    b = gpu.Batch(32, LA, LB)
+    print("fused replays: fused=%s groups=%s" % (b.fused(), [(g["logN"], g["members"]) for g in b.groups()]),
+          flush=True)
+    pa, pb = b.pack(As, Bs)
+    (eager,), _ = _run(b, pa, pb, graph=0)
+    for mode in (1, 2):
+        reps, _ = _run(b, pa, pb, graph=mode, reps=3)
+        for r in reps:
+            for i in range(n):
+                assert np.array_equal(r[i], eager[i]), (mode, i)
+    (again,), _ = _run(b, pa, pb, graph=0, reps=1)
+    for i in range(n):
+        assert np.array_equal(again[i], eager[i]), i
+    _check_vs_gmp(oracle, 32, As, Bs, eager, sample=range(0, n, 11))
