@@ -265,6 +265,7 @@ def bench_batch(n, steps, warmup, seed=4096, host_sample=128, tune=True, compare
         "ms_per_batch_other_graph": {"graph": ["none", "one chain", "a branch per size group"][other],
                                      "ms": other_s * 1e3},
         "launches": int(b.info.nlaunch),
+        "fused_products": b.fused()[0],
         "autotuned": tune,
         "autotune_s": t_tune,
         "groups": [{"logN": g["logN"], "members": g["members"], "passes": g["passes"], "gshift": g["gshift"]}
@@ -321,6 +322,52 @@ def bench_batch(n, steps, warmup, seed=4096, host_sample=128, tune=True, compare
         "same_as_plain_sample": same,
     })
     return res
+
+
+def bench_fused_classes(steps=10):
+    """Batches of equal products, one size class each (32-bit limbs, L x L),
+    through the fused whole-product launch (limit 11) and through stacked
+    transforms (limit 0, autotuned splits), hipGraph replays; both outputs
+    compared limb for limb."""
+    import torch
+    import birdwing_amd as bw
+    out = []
+    for L, n in ((2048, 1024), (1024, 2048), (512, 2048), (256, 4096), (128, 4096), (64, 4096), (16, 8192)):
+        row = {"limbs": L, "products": n}
+        outs = []
+        for lim in (11, 0):
+            old = bw.set_fused_limit(lim)
+            b = bw.Batch(32, [L] * n, [L] * n)
+            bw.set_fused_limit(old)
+            rng = np.random.default_rng(L)
+            da = torch.from_numpy(rng.integers(0, 256, b.bytes_a, dtype=np.uint8)).cuda()
+            db = torch.from_numpy(rng.integers(0, 256, b.bytes_b, dtype=np.uint8)).cuda()
+            dc = torch.zeros(b.bytes_c, dtype=torch.uint8, device="cuda")
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                if lim == 0:
+                    b.tune(da, db, stream=s, reps=2)
+                b.set_graph(1)
+                for _ in range(2):
+                    b.execute(da, db, dc, stream=s)
+                s.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    b.execute(da, db, dc, stream=s)
+                s.synchronize()
+                t = (time.perf_counter() - t0) / steps
+            b.status()
+            nf, lg = b.fused()
+            row["logN"] = lg if nf else [g["logN"] for g in b.groups()][0]
+            row["fused_ms" if lim else "stacked_ms"] = t * 1e3
+            if not lim:
+                row["stacked_passes"] = [g["passes"] for g in b.groups()][0]
+            outs.append(dc.cpu().numpy())
+        row["fused_vs_stacked"] = row["stacked_ms"] / row["fused_ms"]
+        row["same_limbs"] = bool(np.array_equal(outs[0], outs[1]))
+        out.append(row)
+        progress("fused classes: %d x %d limbs done" % (L, L))
+    return {"probe": "fused_classes", "unit": "ms per batch", "steps": steps, "classes": out}
 
 
 class _ServeMix:
@@ -735,6 +782,9 @@ def main():
                     help="only measure the VALU rates of FP64 FMA and 32-bit integer multiplies and exit")
     ap.add_argument("--mfma-dft16", action="store_true",
                     help="only time a 16-point DFT stage on int8 MFMA against the VALU codelet and exit")
+    ap.add_argument("--fused-classes", action="store_true",
+                    help="only time batches of one size class through the fused whole-product launch "
+                         "against stacked transforms and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (--gpus 1); multi-GPU runs are not part of it",
@@ -752,6 +802,9 @@ def main():
             torch.bitwise_not(x, out=y)
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "device copy", "bytes_read": n, "bytes_written": n, "copies": 3}))
+        return
+    if args.fused_classes:
+        print(json.dumps(bench_fused_classes()))
         return
     if args.valu_peak or args.mfma_dft16:
         from birdwing_amd import probes
