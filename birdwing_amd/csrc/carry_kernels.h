@@ -16,6 +16,11 @@
 #pragma once
 #include "bw_internal.h"
 
+#ifndef BW_CARRY_ROWS
+#define BW_CARRY_ROWS 8                                   // rows of 64 limbs per wavefront
+#endif
+#define BW_CARRY_TILE (4 * 64 * BW_CARRY_ROWS)            // limbs per 256-thread workgroup
+
 enum {
     BW_SRC_GARNER_U32 = 0,   // 3 canonical u32 residue planes -> 32-bit limbs
     BW_SRC_GARNER_U64 = 1,   // 3 canonical u64 residue planes -> 64-bit limbs

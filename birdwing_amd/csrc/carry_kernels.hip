@@ -2,8 +2,6 @@
 #include "carry_kernels.h"
 #include "garner_dev.h"
 
-#define BW_CARRY_ROWS 8                                   // rows of 64 limbs per wavefront
-#define BW_CARRY_TILE (4 * 64 * BW_CARRY_ROWS)            // limbs per 256-thread workgroup
 
 // ---------------------------------------------------------------------------
 // Garner's mixed-radix reconstruction x = k0 + p0 k1 + p0 p1 k2 with

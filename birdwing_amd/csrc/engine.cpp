@@ -110,6 +110,21 @@ int get_tables(int set_id, DevTables** out) {
 // pass geometry
 void default_passes(int logN, std::vector<int>& out) {
     out.clear();
+    // Up to 2^16 the autotuner's choices for stacked batches (measured on the
+    // 4096-product mix): one register-staged pass up to 2^10, [7, 4] at 2^11,
+    // two passes with the larger first from 2^12 to 2^16.
+    if (logN <= BW_REG2_MAX_LOG) {
+        out.push_back(logN < 1 ? 1 : logN);
+        return;
+    }
+    if (logN == 11) {
+        out = {7, 4};
+        return;
+    }
+    if (logN <= 16) {
+        out = {(logN + 1) / 2, logN / 2};
+        return;
+    }
     const int P = (logN + 8) / 9 < 1 ? 1 : (logN + 8) / 9;
     const int base = logN / P, extra = logN % P;
     for (int j = 0; j < P; j++) out.push_back(base + (j >= P - extra ? 1 : 0));

@@ -83,8 +83,10 @@ int run_transforms(const Xform& x, const void* d_a, const void* d_b, int64_t la,
                    LaunchMarker* mk);
 int count_transform_launches(const Xform& x);
 
-// Default split of 2^logN into passes of at most 2^9 points, as balanced as
-// possible, the larger passes last (the measured best at 2^25: 8, 8, 9).
+// Default split of 2^logN: one pass up to 2^10, [7, 4] at 2^11, two passes
+// (the larger first) up to 2^16 -- the autotuner's choices for stacked
+// batches --; above, passes of at most 2^9 points, as balanced as possible,
+// the larger last ([8, 8, 8] at 2^24).
 void default_passes(int logN, std::vector<int>& out);
 
 }  // namespace bw

@@ -36,8 +36,8 @@ constexpr size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign;
 constexpr size_t kPlaneBytes = (size_t)kPrimes << BW_NOALLOC_MAX_LOGN << 2;
 constexpr size_t kOffA = 0;
 constexpr size_t kOffB = kPlaneBytes;
-// carry tiles of the largest product: 2^18 + 1 output words in tiles of 2048
-constexpr int64_t kMaxTiles = 256;
+// carry tiles of the largest product: 2^18 + 1 output words
+constexpr int64_t kMaxTiles = (((int64_t)1 << BW_NOALLOC_MAX_LOGN) + 1 + BW_CARRY_TILE - 1) / BW_CARRY_TILE;
 constexpr size_t kOffFlags = 2 * kPlaneBytes;
 constexpr size_t kOffCounter = align_up(kOffFlags + 8 * (size_t)kMaxTiles);   // [0] tile ids, [1] error word
 // (N^-1, N^-1 / p) per prime for each log2 N in [0, BW_NOALLOC_MAX_LOGN]

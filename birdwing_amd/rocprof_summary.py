@@ -2,6 +2,9 @@
 
   python -m birdwing_amd.rocprof_summary stats DIR        kernel-trace --stats table (markdown)
   python -m birdwing_amd.rocprof_summary pmc DIR          per-kernel average of each PMC counter
+  python -m birdwing_amd.rocprof_summary last DIR K       the last K kernel dispatches of a kernel
+        trace in start order (the runtime's own copy and fill kernels left out): duration,
+        gap since the previous one ended, grid
   python -m birdwing_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR CALIB_MIB LABEL=REGEX ...
         HBM bytes per dispatch of each labelled kernel (kernels matching REGEX), from a
         FETCH_SIZE run and a WRITE_SIZE run (KiB per dispatch), each scaled by what the
@@ -103,11 +106,34 @@ def traffic(fetch_dir, write_dir, calib_mib, pairs):
     return out
 
 
+def last(d, k):
+    rows = [r for r in _rows(d, "*kernel_trace.csv") if not _get(r, "Kernel_Name").startswith("__amd_rocclr")]
+    rows.sort(key=lambda r: int(_get(r, "Start_Timestamp")))
+    rows = rows[-k:]
+    lines = ["| # | kernel | us | gap us | grid |", "|---|---|---|---|---|"]
+    prev = None
+    for i, r in enumerate(rows):
+        s, e = int(_get(r, "Start_Timestamp")), int(_get(r, "End_Timestamp"))
+        gap = (s - prev) / 1e3 if prev is not None else 0.0
+        prev = e
+        grid = "%s x %s x %s / %s" % (_get(r, "Grid_Size_X"), _get(r, "Grid_Size_Y"), _get(r, "Grid_Size_Z"),
+                                      _get(r, "Workgroup_Size_X"))
+        lines.append("| %d | %s | %.1f | %.1f | %s |" % (i, short(_get(r, "Kernel_Name"), 60), (e - s) / 1e3, gap, grid))
+    span = (int(_get(rows[-1], "End_Timestamp")) - int(_get(rows[0], "Start_Timestamp"))) / 1e3
+    busy = sum(int(_get(r, "End_Timestamp")) - int(_get(r, "Start_Timestamp")) for r in rows) / 1e3
+    lines.append("")
+    lines.append("%d dispatches: %.1f us from the first start to the last end, %.1f us inside kernels" %
+                 (len(rows), span, busy))
+    return "\n".join(lines)
+
+
 def main(argv):
     if len(argv) >= 2 and argv[0] == "stats":
         print(stats(argv[1]))
     elif len(argv) >= 2 and argv[0] == "pmc":
         print(pmc(argv[1]))
+    elif len(argv) >= 3 and argv[0] == "last":
+        print(last(argv[1], int(argv[2])))
     elif len(argv) >= 5 and argv[0] == "traffic":
         pairs = [a.rsplit("=", 1) for a in argv[4:]]   # labels may hold "=", regexes do not
         print(json.dumps(traffic(argv[1], argv[2], float(argv[3]), pairs), indent=1))
