@@ -43,15 +43,13 @@ int fused_limit() { return g_fused_limit.load(); }
 
 // Workgroups of a fused launch: one per CU (its LDS and registers hold one).
 static int fused_grid(int nbundles) {
-    static int cus = 0;
-    if (!cus) {
+    static const int cus = [] {
         int dev = 0, c = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && c > 0)
-            cus = c;
-        else
-            return std::min(nbundles, 256);
-    }
+            return c;
+        return 256;
+    }();
     return std::min(nbundles, cus);
 }
 
