@@ -889,6 +889,23 @@ def main():
         legs["three_prime_u64"] = f64b
         progress("64-bit-limb legs timed")
         del A64, B64, o64a, o64b
+        # the small plain product (2048 x 2048 limbs, the reference's 32-bit
+        # prime set, N = 4096: latency-bound, so p50/p99 per product) and the
+        # unbalanced carry-stress shape (2^22 x 2^14 limbs)
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        for name, la, lb, ps, steps in (("small_2048x2048_u32_ref", 2048, 2048, bw.SET_U32_REF, 500),
+                                        ("small_2048x2048_u32", 2048, 2048, -1, 500),
+                                        ("stress_4194304x16384_u32", 1 << 22, 1 << 14, -1, 20)):
+            As = synthetic_limbs(la, 5, 32)
+            Bs = synthetic_limbs(lb, 6, 32)
+            fs, outs = bench_plain(As, Bs, 32, ps, None, -1, steps, 20, not args.no_tune, name)
+            fs["method"] = method_of(fs)
+            if pyoracle.gmp_available():
+                g, _ = pyoracle.gmp_mul(As, Bs, 32)
+                fs["bit_exact_vs_gmp"] = bool(np.array_equal(g, outs))
+            legs[name] = fs
+            progress("%s timed (p50 %.3f ms)" % (name, fs["latency_ms"]["p50"]))
         line["legs"] = legs
     torch.cuda.empty_cache()
     if args.batch_n > 0:
