@@ -151,3 +151,20 @@ def test_abi_version_matches_header():
     hdr = open(os.path.join(ROOT, "include", "birdwing.h")).read()
     v = int(re.search(r"#define BW_ABI_VERSION (\d+)", hdr).group(1))
     assert bw.lib().bw_abi_version() == v
+
+
+def test_fused_limit_setter():
+    """bw_set_fused_limit: 0 .. 11 accepted (returns the previous limit, 8
+    at start), anything else refused with BW_ERR_ARG; host only."""
+    import birdwing_amd as bw
+    L = bw.lib()
+    first = L.bw_set_fused_limit(5)
+    assert first == 8
+    assert L.bw_set_fused_limit(11) == 5
+    assert L.bw_set_fused_limit(12) == bw.BW_ERR_ARG
+    assert L.bw_set_fused_limit(-1) == bw.BW_ERR_ARG
+    assert "fused limit" in L.bw_last_error().decode()
+    assert L.bw_set_fused_limit(0) == 11
+    assert bw.set_fused_limit(first) == 0
+    with pytest.raises(bw.BWError):
+        bw.set_fused_limit(40)

@@ -313,3 +313,35 @@ def test_fused_graph_replays(gpu, oracle):
     for i in range(n):
         assert np.array_equal(again[i], eager[i]), i
     _check_vs_gmp(oracle, 32, As, Bs, eager, sample=range(0, n, 11))
+
+
+def test_fused_mixed_widths_in_bundles(gpu, oracle):
+    """32- and 64-bit products of the same transform lengths share bundles of
+    the fused launch (several products per workgroup, each with its own limb
+    width, lengths and output offset): every product against GMP."""
+    rng = np.random.default_rng(57)
+    specs = []
+    for L in range(3, 10):
+        n = 1 << L
+        for _ in range(6):
+            bits = int(rng.choice([32, 64]))
+            per = 2 if bits == 32 else 1
+            da = int(rng.integers(1, n // 2 + 1))
+            db = int(rng.integers(1, n - da + 2))
+            specs.append((bits, per * da - int(rng.integers(0, per)), per * db))
+    bits = [s[0] for s in specs]
+    LA = [max(1, s[1]) for s in specs]
+    LB = [max(1, s[2]) for s in specs]
+    As = [oracle.uniform_limbs(la, 700 + i, bt) for i, (bt, la) in enumerate(zip(bits, LA))]
+    Bs = [oracle.uniform_limbs(lb, 800 + i, bt) for i, (bt, lb) in enumerate(zip(bits, LB))]
+    old = gpu.set_fused_limit(11)
+    try:
+        b = gpu.Batch(bits, LA, LB)
+    finally:
+        gpu.set_fused_limit(old)
+    nf, _ = b.fused()
+    print("fused mixed widths: %d products, %d fused, groups %s" % (len(specs), nf, b.groups()), flush=True)
+    assert nf == len(specs)
+    pa, pb = b.pack(As, Bs)
+    (got,), _ = _run(b, pa, pb)
+    _check_vs_gmp(oracle, bits, As, Bs, got)
