@@ -608,6 +608,7 @@ extern "C" int bw_forward(bw_plan* p, const void* d_raw, uint64_t L, void* d_spe
         if (j == P - 1 && g.S == 1 && P == 1) g.G = 1;
         PassParams prm = base_params(x, g);
         prm.nops = 1;
+        prm.canon_out = j == P - 1;   // canonical spectrum out; the passes before it store centered words
         if (j == 0) {
             prm.ingress = p->ingress;
             prm.in[0] = d_raw;

@@ -39,6 +39,23 @@ __device__ __forceinline__ double f64_mulmod(double y, double w, double wp, cons
     return __builtin_fma(y, w, -t) - q;
 }
 
+// y * w mod p straight to a 32-bit word in (-p, p): four FP64 operations and
+// no conversion.  With M = 1.5*2^52:
+//   qr = fma(y, -w/p, M)         = M + q, q = rint(-y*w/p)
+//   t  = fma(qr, p-1, -M*(p-1))  = q*(p-1) exactly (q*c < 2^53, M*(p-1) exact)
+//   u  = fma(y, -w, -t)          = r + q with r = -y*w - q*p, exactly as above
+//   v  = u - qr                  = r - M, exactly (an integer in (-2^53, -2^52))
+// v's low 32 significand bits are (2^51 - r) mod 2^32 = -r mod 2^32, so the
+// low word read as int32 is -r = y*w + q*p, congruent to y*w (|r| < p < 2^31).
+// The negations are FMA source modifiers.  Same input range as f64_mulmod.
+__device__ __forceinline__ int32_t f64_mulmod_i32(double y, double w, double wp, const F64Prime& P) {
+    const double qr = __builtin_fma(y, -wp, BW_MAGIC_RND);
+    const double t = __builtin_fma(qr, P.cm, -(BW_MAGIC_RND * P.cm));
+    const double u = __builtin_fma(y, -w, -t);
+    const double v = u - qr;
+    return (int32_t)(uint32_t)__builtin_bit_cast(uint64_t, v);
+}
+
 // Reduce |x| < 2^51 to a centered representative in [-p/2-1, p/2+1].
 __device__ __forceinline__ double f64_reduce(double x, const F64Prime& P) {
     double q = __builtin_fma(x, P.pinv, BW_MAGIC_RND) - BW_MAGIC_RND;

@@ -83,7 +83,9 @@ struct PassParams {
     void* out[2];
     int64_t in_len[2];     // raw limb counts when ingress != 0
     int ingress;           // 1: inputs are raw limbs (zero-padded to N, pass 1 only)
-    int canon_out;         // 1: store canonical residues in [0, p); 0: centered (32-bit path)
+    int canon_out;         // 1: store canonical residues in [0, p); 0: centered (32-bit path).
+                           // Forward column passes of the register-staged kernels store
+                           // centered residues whatever this says (never a transform's last pass).
     int fold_ninv;         // FUSED: skip N^-1 (the next INV pass applies it); INV: apply N^-1
     int nops;              // FWD: operands in this launch
     int64_t N;             // transform length
@@ -98,6 +100,9 @@ struct PassParams {
     const void* tw_hi;
     const void* ninv;      // per-prime (N^-1, aux) pairs for FUSED mode
     const BwIngress* bdesc;  // ingress of stacked products (null: one product, in_len)
+    int rev;               // 1: the launch walks its workgroups' tiles, primes and operands in
+                           // reverse order (reads first what the launch before it wrote last,
+                           // while the Infinity Cache still holds it)
 };
 
 // Status codes (BW_OK, BW_ERR_*) come from the public header.

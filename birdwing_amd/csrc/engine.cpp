@@ -253,6 +253,13 @@ int run_transforms(const Xform& x, const void* d_a, const void* d_b, int64_t la,
     const void* raw[2] = {d_a, d_b};
     void* buf[2] = {x.bufA, x.bufB};
     const int64_t len[2] = {la, lb};
+    // Each launch walks its workgroups in the direction opposite to the
+    // launch before it, so it first reads what that launch wrote last: a
+    // pass over 2^24-point planes then takes part of its input from the
+    // 256 MB Infinity Cache (measured on a memory-only copy of the second
+    // forward pass: 270 instead of 290 us).
+    int walk = 0;
+    auto next_rev = [&]() { return BW_ALTERNATE_WALK ? (walk++ & 1) : 0; };
     // forward passes 1 .. P-1 of primes [prime0, prime0 + nper): one operand
     // (op 0 or 1) or both at once (op < 0)
     auto forward = [&](int prime0, int nper, int op) -> int {
@@ -271,6 +278,7 @@ int run_transforms(const Xform& x, const void* d_a, const void* d_b, int64_t la,
             if (j == 0) prm.bdesc = x.bdesc;
             prm.ingress = j == 0 ? x.ingress : 0;
             prm.nops = nops;
+            prm.rev = next_rev();
             PassLaunch L = {BW_MODE_FWD, g.logT, g.G, x.limb_bits, nper, nops, g.tiles};
             LAUNCH_TRY(bw_launch_pass(L, prm, st));
             MARK();
@@ -288,6 +296,7 @@ int run_transforms(const Xform& x, const void* d_a, const void* d_b, int64_t la,
             prm.out[0] = x.bufA;
             prm.canon_out = 0;
             prm.fold_ninv = folded;
+            prm.rev = next_rev();
             PassLaunch L = {BW_MODE_FUSED, g.logT, g.G, x.limb_bits, nper, 1, g.tiles};
             LAUNCH_TRY(bw_launch_pass(L, prm, st));
             MARK();
@@ -300,6 +309,7 @@ int run_transforms(const Xform& x, const void* d_a, const void* d_b, int64_t la,
             prm.out[0] = x.bufA;
             prm.canon_out = j == 0 ? final_canon : 0;
             prm.fold_ninv = (j == P - 2) && folded;
+            prm.rev = next_rev();
             PassLaunch L = {BW_MODE_INV, g.logT, g.G, x.limb_bits, nper, 1, g.tiles};
             LAUNCH_TRY(bw_launch_pass(L, prm, st));
             MARK();

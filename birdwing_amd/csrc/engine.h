@@ -11,6 +11,12 @@
 #include "host_math.h"
 #include "ntt_kernels.h"
 
+// 1: the passes of a product alternate the direction of their workgroup walk
+// (PassParams::rev); 0: every pass walks forward.
+#ifndef BW_ALTERNATE_WALK
+#define BW_ALTERNATE_WALK 1
+#endif
+
 namespace bw {
 
 // Records the message for bw_last_error() (thread-local) and returns `code`.

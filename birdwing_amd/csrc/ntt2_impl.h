@@ -45,6 +45,10 @@ struct ArF64x {
     // reduction before the 32-bit conversion
     __device__ static LV to_lds_r(V x) { return (int32_t)x; }
     __device__ static W to_word_r(V x) { return (uint32_t)(int32_t)x; }
+    // a twiddle product reduced straight to the 32-bit LDS / storage word
+    // (four FP64 operations and no conversion, f64_mulmod_i32)
+    __device__ static LV lds_mul(V x, TW w, const PR& P) { return f64_mulmod_i32(x, w.x, w.y, P); }
+    __device__ static W word_mul(V x, double w, const PR& P) { return (uint32_t)f64_mulmod_i32(x, w, w * P.pinv, P); }
     // a*b (no N^-1) for a, b < 2^46 in magnitude
     __device__ static V mul_plain(V a, V b, const PR& P) {
         const double br = f64_reduce(b, P);
@@ -72,11 +76,11 @@ struct ArF64x {
     __device__ static V from_keep(KV k) { return (double)k; }
     __device__ static KV keep_mul(KV k, V x, const PR& P) {
         const double w = (double)k;
-        return (int32_t)f64_mulmod(x, w, w * P.pinv, P);
+        return f64_mulmod_i32(x, w, w * P.pinv, P);
     }
     __device__ static KV keep_mul_scaled(KV k, V x, const NI* ninv, const PR& P) {
         const double w = (double)k;
-        return (int32_t)f64_mulmod(f64_mulmod(x, w, w * P.pinv, P), ninv[0], ninv[1], P);
+        return f64_mulmod_i32(f64_mulmod(x, w, w * P.pinv, P), ninv[0], ninv[1], P);
     }
     struct Gen {          // w_k = base * step^k, values in (-p, p)
         double w, step, stepp;
@@ -144,6 +148,8 @@ struct ArU64x {
     __device__ static W to_word_c(V x, const PR&, bool) { return x; }
     __device__ static LV to_lds_r(V x) { return x; }
     __device__ static W to_word_r(V x) { return x; }
+    __device__ static LV lds_mul(V x, TW w, const PR& P) { return mul(x, w, P); }
+    __device__ static W word_mul(V x, uint64_t w, const PR& P) { return apply_tw(x, w, P); }
     // montmul(montmul(a, b), 2^128) = a*b
     __device__ static V mul_plain(V a, V b, const PR& P) { return u64_montmul(u64_montmul(a, b, P), P.r2, P); }
     // ninv[0] = N^-1 2^128 mod p: montmul(t, ninv) = t N^-1 2^64 -> montmul by 1 -> t N^-1
@@ -309,7 +315,7 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
     const int G = 1 << logG;
     // stage twiddles w_T^e (e < T) staged once per workgroup after the tile
     TW* twl = reinterpret_cast<TW*>(smem_raw + t2_tile_bytes(G, LT, (int)sizeof(typename AR::LV)));
-    const int op = (MODE == BW_MODE_FWD) ? blockIdx.z : 0;
+    const int op = (MODE == BW_MODE_FWD) ? (prm.rev ? (int)gridDim.z - 1 - (int)blockIdx.z : (int)blockIdx.z) : 0;
     // read-only inputs of the launch through the constant address space, so
     // they load as scalars however the kernel's stores might alias them
     const typename AR::PC32 pc = bw_ldconst(reinterpret_cast<const typename AR::PC32*>(prm.consts), prime);
@@ -518,11 +524,9 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
             dft32_dif<AR, R0LOG>(x, pc);
         }
 #pragma unroll
-        for (int k = 1; k < R0; k++) x[brev(k, R0LOG)] = AR::mul(x[brev(k, R0LOG)], twl[j0 * k], P);
-#pragma unroll
-        for (int k = 0; k < R0; k++)   // k > 0 left a twiddle product in (-p, p)
+        for (int k = 0; k < R0; k++)   // k > 0: the stage twiddle product goes straight to the LDS word
             lds[g0 * LT + T2Lds<LOGT, COLS>::pos_kj(k, j0)] =
-                k == 0 ? AR::to_lds(x[0], P) : AR::to_lds_r(x[brev(k, R0LOG)]);
+                k == 0 ? AR::to_lds(x[0], P) : AR::lds_mul(x[brev(k, R0LOG)], twl[j0 * k], P);
     };
     auto fwd_stage0 = [&](int which) {
         IW w[R0];
@@ -575,21 +579,17 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
             };
             if constexpr (COLS) {
                 const int64_t s = s0 + g;
-                auto twx = [&](int k, auto w) { x[brev(k, R1LOG)] = AR::apply_tw(x[brev(k, R1LOG)], w, P); };
-                if constexpr (CTAB)
-                    AR::template for_powers<R1>(AR::twprod(utab[b], ztab[b * g], P), AR::twprod(utab[R0], ztab[R0 * g], P),
-                                                P, twx);
-                else
-                    AR::template for_powers<R1>(tw_lookup<AR>(((int64_t)b * s) << twshift, twlo, twhi, pc),
-                                                tw_lookup<AR>(((int64_t)R0 * s) << twshift, twlo, twhi, pc), P, twx);
-            }
-            // one uniform branch around the stores: no per-element select of both forms
-            if (canon) {
+                const TW base = CTAB ? AR::twprod(utab[b], ztab[b * g], P)
+                                     : tw_lookup<AR>(((int64_t)b * s) << twshift, twlo, twhi, pc);
+                const TW step = CTAB ? AR::twprod(utab[R0], ztab[R0 * g], P)
+                                     : tw_lookup<AR>(((int64_t)R0 * s) << twshift, twlo, twhi, pc);
+                // each pass-twiddle product goes straight to its storage word
+                // (centered: a forward column pass is never a transform's last)
+                AR::template for_powers<R1>(base, step, P,
+                                            [&](int k, auto w) { put(k, AR::word_mul(x[brev(k, R1LOG)], w, P)); });
+            } else if (canon) {
 #pragma unroll
                 for (int k = 0; k < R1; k++) put(k, AR::to_word_c(x[brev(k, R1LOG)], P, true));
-            } else if constexpr (COLS) {      // every value left a pass-twiddle product in (-p, p)
-#pragma unroll
-                for (int k = 0; k < R1; k++) put(k, AR::to_word_r(x[brev(k, R1LOG)]));
             } else {
 #pragma unroll
                 for (int k = 0; k < R1; k++) put(k, AR::to_word_c(x[brev(k, R1LOG)], P, false));
@@ -653,8 +653,7 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
             lds[g * LT + T2Lds<LOGT, COLS>::pos(b * R1)] = AR::to_lds(x[0], P);
 #pragma unroll
             for (int m = 1; m < R1; m++)
-                lds[g * LT + T2Lds<LOGT, COLS>::pos(b * R1 + m)] =
-                    AR::to_lds_r(AR::mul(x[m], twl[(T - m * b) & (T - 1)], P));
+                lds[g * LT + T2Lds<LOGT, COLS>::pos(b * R1 + m)] = AR::lds_mul(x[m], twl[(T - m * b) & (T - 1)], P);
         }
     };
     // stage 0 inverse: LDS (stage twiddles already applied) -> inverse DFT_R0 -> HBM
@@ -746,7 +745,9 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
 // walks the tiles of one prime after another (blockIdx.x fastest: the raw
 // operands, read by every prime, stream through in order once per prime);
 // the other passes take the primes of a tile back to back, which measured
-// 3-7% faster for them.
+// 3-7% faster for them.  prm.rev reverses the whole walk (last tile, prime
+// and operand first), so that a pass starts on the data its predecessor
+// wrote last.
 template <class AR, int LOGT, int MODE, bool COLS, int INGRESS>
 constexpr int ntt2_max_threads() {
     return (MODE == BW_MODE_INV && COLS && t2_inv_wide_ok(LOGT, (int)sizeof(typename AR::LV))) ? 1024 : 512;
@@ -754,11 +755,13 @@ constexpr int ntt2_max_threads() {
 
 template <class AR, int LOGT, int MODE, bool COLS, int INGRESS>
 __global__ __launch_bounds__((ntt2_max_threads<AR, LOGT, MODE, COLS, INGRESS>())) void ntt2_kernel(PassParams prm) {
+    const uint32_t bx = prm.rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+    const uint32_t by = prm.rev ? gridDim.y - 1 - blockIdx.y : blockIdx.y;
     if constexpr (INGRESS == 0) {
-        const uint32_t id = blockIdx.x + blockIdx.y * gridDim.x;
+        const uint32_t id = bx + by * gridDim.x;
         ntt2_body<AR, LOGT, MODE, COLS, INGRESS>(prm, (int)(id / gridDim.y), prm.prime0 + (int)(id % gridDim.y));
     } else {
-        ntt2_body<AR, LOGT, MODE, COLS, INGRESS>(prm, blockIdx.x, prm.prime0 + blockIdx.y);
+        ntt2_body<AR, LOGT, MODE, COLS, INGRESS>(prm, (int)bx, prm.prime0 + (int)by);
     }
 }
 
