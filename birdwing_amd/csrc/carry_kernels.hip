@@ -69,7 +69,21 @@ struct CarryView {
 };
 // WAVES: the least waves per SIMD the kernel is compiled for (a register
 // cap; each source at the most that compiles without spills).
-struct SrcGarnerU32 {
+//
+// fetch() issues the loads of coefficient n into a Fetch record and words()
+// turns the record into limb words, so the kernel can put the loads of all
+// its rows in flight before the first conversion.  LateFetch: the record is
+// the index, and words() loads and converts.
+template <class S>
+struct LateFetch {
+    struct Fetch { int64_t n; };
+    __device__ static void fetch(const CarryView&, int64_t n, Fetch& f) { f.n = n; }
+    template <class Wd>
+    __device__ static void words(const CarryView& p, const CarryParams& c, int64_t, const Fetch& f, Wd* w) {
+        S::load(p, c, f.n, w);
+    }
+};
+struct SrcGarnerU32 : LateFetch<SrcGarnerU32> {
     using Word = uint32_t;
     static constexpr int NW = 3;
     static constexpr int WAVES = 5;
@@ -82,7 +96,7 @@ struct SrcGarnerU32 {
         w[0] = in ? (uint32_t)lo : 0u; w[1] = in ? (uint32_t)(lo >> 32) : 0u; w[2] = in ? hi : 0u;
     }
 };
-struct SrcGarnerU64 {
+struct SrcGarnerU64 : LateFetch<SrcGarnerU64> {
     using Word = uint64_t;
     static constexpr int NW = 3;
     static constexpr int WAVES = 4;
@@ -94,7 +108,7 @@ struct SrcGarnerU64 {
         if (!in) w[0] = w[1] = w[2] = 0;
     }
 };
-struct SrcGarnerU64x2 {
+struct SrcGarnerU64x2 : LateFetch<SrcGarnerU64x2> {
     using Word = uint32_t;
     static constexpr int NW = 4;
     static constexpr int WAVES = 5;
@@ -111,18 +125,28 @@ struct SrcGarnerU64x2 {
 struct SrcGarnerD64 {
     using Word = uint64_t;
     static constexpr int NW = 3;
-    static constexpr int WAVES = 5;
-    __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
+    static constexpr int WAVES = 4;   // room for the residues of all rows in flight
+    // the five residue words of one coefficient (index clamped into range:
+    // the loads carry no branch)
+    struct Fetch { uint32_t r[5]; };
+    __device__ static void fetch(const CarryView& p, int64_t n, Fetch& f) {
+        typedef const __attribute__((address_space(1))) uint32_t GU32;
         const bool in = n >= 0 && n < p.N;
-        const uint32_t* a = reinterpret_cast<const uint32_t*>(p.a0) + (in ? n : 0);
-        uint32_t r[5];
+        const GU32* a = (const GU32*)p.a0 + (in ? n : 0);
 #pragma unroll
-        for (int j = 0; j < 5; j++) r[j] = a[j * p.plane];
-        garner_d64(r, c.gd, w);
-        if (!in) w[0] = w[1] = w[2] = 0;
+        for (int j = 0; j < 5; j++) f.r[j] = a[j * p.plane];
+    }
+    __device__ static void words(const CarryView& p, const CarryParams& c, int64_t n, const Fetch& f, Word* w) {
+        garner_d64(f.r, c.gd, w);
+        if (!(n >= 0 && n < p.N)) w[0] = w[1] = w[2] = 0;
+    }
+    __device__ static void load(const CarryView& p, const CarryParams& c, int64_t n, Word* w) {
+        Fetch f;
+        fetch(p, n, f);
+        words(p, c, n, f, w);
     }
 };
-struct SrcHiLo128 {     // a0 = hi, a1 = lo
+struct SrcHiLo128 : LateFetch<SrcHiLo128> {     // a0 = hi, a1 = lo
     using Word = uint32_t;
     static constexpr int NW = 4;
     static constexpr int WAVES = 5;
@@ -133,7 +157,7 @@ struct SrcHiLo128 {     // a0 = hi, a1 = lo
         w[0] = (uint32_t)lo; w[1] = (uint32_t)(lo >> 32); w[2] = (uint32_t)hi; w[3] = (uint32_t)(hi >> 32);
     }
 };
-struct SrcHiLo128L64 {  // a0 = hi, a1 = lo, 64-bit limbs
+struct SrcHiLo128L64 : LateFetch<SrcHiLo128L64> {  // a0 = hi, a1 = lo, 64-bit limbs
     using Word = uint64_t;
     static constexpr int NW = 2;
     static constexpr int WAVES = 5;
@@ -143,7 +167,7 @@ struct SrcHiLo128L64 {  // a0 = hi, a1 = lo, 64-bit limbs
         w[0] = reinterpret_cast<const uint64_t*>(p.a1)[n];
     }
 };
-struct SrcU160 {        // a0 = lo, a1 = mid, a2 = hi32
+struct SrcU160 : LateFetch<SrcU160> {        // a0 = lo, a1 = mid, a2 = hi32
     using Word = uint64_t;
     static constexpr int NW = 3;
     static constexpr int WAVES = 5;
@@ -235,16 +259,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SRC::WAVES)
     Wd zero[NW];
 #pragma unroll
     for (int j = 0; j < NW; j++) zero[j] = 0;
+    // the loads of every row (and, in wave 0, of the previous tile's last
+    // row) are issued before any coefficient is converted
+    typename SRC::Fetch fprev;
+    if (wave == 0) SRC::fetch(view, wbase - 64 + lane, fprev);
+    typename SRC::Fetch fr[K];
+#pragma unroll
+    for (int i = 0; i < K; i++) SRC::fetch(view, wbase + i * 64 + lane, fr[i]);
     if (wave == 0) {
         // the previous tile's last 64 limbs (zero before the product's start)
         Wd cw[NW];
-        SRC::load(view, prm, wbase - 64 + lane, cw);
+        SRC::words(view, prm, wbase - 64 + lane, fprev, cw);
         Wd lo, hi;
         row_sums(cw, zero, lo, hi);
         publish_tail(0, cw, hi);
     }
     Wd cw0[NW];
-    SRC::load(view, prm, wbase + lane, cw0);
+    SRC::words(view, prm, wbase + lane, fr[0], cw0);
     Wd pw[NW];
     Wd phi;
     {
@@ -257,7 +288,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SRC::WAVES)
 #pragma unroll
     for (int i = 1; i < K; i++) {
         Wd cw[NW];
-        SRC::load(view, prm, wbase + i * 64 + lane, cw);
+        SRC::words(view, prm, wbase + i * 64 + lane, fr[i], cw);
         Wd lo, hi;
         row_sums(cw, pw, lo, hi);
         const Wd h1 = shift_up(hi, phi, 1, lane);
