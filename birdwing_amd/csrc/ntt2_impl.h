@@ -523,10 +523,15 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
             }
             dft32_dif<AR, R0LOG>(x, pc);
         }
+        // k > 0: the stage twiddle product goes straight to the LDS word.  All
+        // products first, then all writes: the twiddle reads share the LDS
+        // buffer with the tile, so a write between two reads would order them.
+        typename AR::LV v[R0];
+        v[0] = AR::to_lds(x[0], P);
 #pragma unroll
-        for (int k = 0; k < R0; k++)   // k > 0: the stage twiddle product goes straight to the LDS word
-            lds[g0 * LT + T2Lds<LOGT, COLS>::pos_kj(k, j0)] =
-                k == 0 ? AR::to_lds(x[0], P) : AR::lds_mul(x[brev(k, R0LOG)], twl[j0 * k], P);
+        for (int k = 1; k < R0; k++) v[k] = AR::lds_mul(x[brev(k, R0LOG)], twl[j0 * k], P);
+#pragma unroll
+        for (int k = 0; k < R0; k++) lds[g0 * LT + T2Lds<LOGT, COLS>::pos_kj(k, j0)] = v[k];
     };
     auto fwd_stage0 = [&](int which) {
         IW w[R0];
@@ -650,10 +655,13 @@ __device__ __forceinline__ void ntt2_body(const PassParams& prm, int bx, int pri
             // index b) is applied here: its product leaves the value reduced.
             // m = 0 is untwiddled (a reduction); b = 0 multiplies by twl[0] = 1,
             // which keeps the loop free of branches.
-            lds[g * LT + T2Lds<LOGT, COLS>::pos(b * R1)] = AR::to_lds(x[0], P);
+            // all products (their twiddle reads) before the LDS writes
+            typename AR::LV v[R1];
+            v[0] = AR::to_lds(x[0], P);
 #pragma unroll
-            for (int m = 1; m < R1; m++)
-                lds[g * LT + T2Lds<LOGT, COLS>::pos(b * R1 + m)] = AR::lds_mul(x[m], twl[(T - m * b) & (T - 1)], P);
+            for (int m = 1; m < R1; m++) v[m] = AR::lds_mul(x[m], twl[(T - m * b) & (T - 1)], P);
+#pragma unroll
+            for (int m = 0; m < R1; m++) lds[g * LT + T2Lds<LOGT, COLS>::pos(b * R1 + m)] = v[m];
         }
     };
     // stage 0 inverse: LDS (stage twiddles already applied) -> inverse DFT_R0 -> HBM
