@@ -32,7 +32,7 @@ Reported beside products/s:
     thread).
 
 `--gpus N` with N > 1 exits: this build measures one GPU.
-`--valu-peak` and `--mfma-dft16` run the arithmetic-rate probes instead.
+`--valu-peak`, `--mfma-dft16` and `--hbm-peak` run the rate probes instead.
 """
 import argparse
 import json
@@ -782,6 +782,9 @@ def main():
                     help="only measure the VALU rates of FP64 FMA and 32-bit integer multiplies and exit")
     ap.add_argument("--mfma-dft16", action="store_true",
                     help="only time a 16-point DFT stage on int8 MFMA against the VALU codelet and exit")
+    ap.add_argument("--hbm-peak", action="store_true",
+                    help="only measure HBM copy/read/write streaming rates and a column pass's memory pattern "
+                         "without arithmetic, and exit")
     ap.add_argument("--fused-classes", action="store_true",
                     help="only time batches of one size class through the fused whole-product launch "
                          "against stacked transforms and exit")
@@ -806,9 +809,12 @@ def main():
     if args.fused_classes:
         print(json.dumps(bench_fused_classes()))
         return
-    if args.valu_peak or args.mfma_dft16:
+    if args.valu_peak or args.mfma_dft16 or args.hbm_peak:
         from birdwing_amd import probes
-        print(json.dumps(probes.valu_rates() if args.valu_peak else probes.mfma_dft16()))
+        if args.hbm_peak:
+            print(json.dumps(probes.hbm_rates()))
+        else:
+            print(json.dumps(probes.valu_rates() if args.valu_peak else probes.mfma_dft16()))
         return
 
     L = args.limbs

@@ -125,9 +125,17 @@ void default_passes(int logN, std::vector<int>& out) {
         out = {(logN + 1) / 2, logN / 2};
         return;
     }
+    // Above 2^16, passes of at most 2^9 points; the odd points go to the
+    // middle passes first, then the first: a last (row, fused) pass of 2^9
+    // points measured 1.7x slower than one of 2^8 (482 against 278 us at
+    // 2^24), a middle column pass of 2^9 only ~8% slower per point.
     const int P = (logN + 8) / 9 < 1 ? 1 : (logN + 8) / 9;
-    const int base = logN / P, extra = logN % P;
-    for (int j = 0; j < P; j++) out.push_back(base + (j >= P - extra ? 1 : 0));
+    const int base = logN / P;
+    int extra = logN % P;
+    out.assign(P, base);
+    for (int j = 1; j < P - 1 && extra > 0; j++, extra--) out[j]++;
+    for (int j = 0; j < P - 1 && extra > 0; j++, extra--) out[j]++;
+    if (extra > 0) out[P - 1]++;
 }
 
 PassPlan pass_geom(const Xform& x, int j) {

@@ -30,6 +30,8 @@ def lib():
         L = C.CDLL(PROBE_PATH)
         L.bwp_valu_rate.restype = C.c_double
         L.bwp_valu_rate.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.bwp_hbm_rate.restype = C.c_double
+        L.bwp_hbm_rate.argtypes = [C.c_int, C.c_int64, C.c_int, C.c_int]
         L.bwp_mfma_dft16.restype = C.c_int
         L.bwp_mfma_dft16.argtypes = [C.c_int64, C.c_int, C.c_int, C.POINTER(C.c_double), C.c_void_p, C.c_void_p,
                                      C.c_int]
@@ -103,3 +105,38 @@ def mfma_dft16(lines=1 << 22, reps=5, rounds=(1, 8), nsample=32):
     out["config"] = {"reps": reps, "lines_per_mfma_block": 16, "mfma": "v_mfma_i32_16x16x64_i8",
                      "mfmas_per_block_and_round": 7}
     return out
+
+
+HBM_PEAK = 8.0e12   # MI355X published HBM3E bandwidth, bytes/s
+
+
+def hbm_rates(mib=640, reps=10):
+    """Device streaming rates in GB/s (best of `reps` runs): copy, read and
+    write of `mib` MiB at three grid sizes, and a 256-point column pass of
+    stride 2^8 over ten 2^24-word planes (the memory pattern of the middle
+    forward pass, no arithmetic) walked forward and in reverse right after
+    a pass of stride 2^16 over the same planes."""
+    import torch
+    torch.cuda.init()
+    out = {}
+    for kind, name in ((0, "copy"), (1, "read"), (2, "write")):
+        for grid in (1024, 4096, 16384):
+            r = float(lib().bwp_hbm_rate(kind, mib, grid, reps))
+            if r < 0:
+                raise RuntimeError("hbm probe %s failed" % name)
+            out["%s_grid%d" % (name, grid)] = r
+    for kind, name in ((3, "colpass_s256_forward_walk"), (4, "colpass_s256_reverse_walk")):
+        r = float(lib().bwp_hbm_rate(kind, 0, 1, reps))
+        if r < 0:
+            raise RuntimeError("hbm probe %s failed" % name)
+        out[name] = r
+    best = {k: max(v for kk, v in out.items() if kk.startswith(k + "_grid")) for k in ("copy", "read", "write")}
+    return {
+        "probe": "hbm_rates",
+        "unit": "GB/s (1e9 bytes moved per second: read + written)",
+        "rates": out,
+        "best": best,
+        "peak_GBps": HBM_PEAK / 1e9,
+        "best_frac_of_peak": {k: v / (HBM_PEAK / 1e9) for k, v in best.items()},
+        "config": {"MiB": mib, "threads": 256, "reps": reps, "colpass_bytes": 2 * 10 * (1 << 24) * 4},
+    }

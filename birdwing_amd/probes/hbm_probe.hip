@@ -1,0 +1,117 @@
+// hbm_probe.hip -- HBM streaming rates of the device (measurement only).
+//
+// What the pass kernels' memory side can expect on this part: a plain copy,
+// reads alone, writes alone, and the exact access pattern of a 256-point
+// column pass over ten 2^24-word planes (32-column tiles of 4-byte words,
+// 16 loads per thread, one padded LDS transpose, 16 stores per thread) with
+// no arithmetic, walked forward or in reverse after a pass of the other
+// stride (the reverse walk reads first what the pass before it wrote last).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace {
+
+__global__ __launch_bounds__(256) void hp_copy(const uint4* __restrict__ in, uint4* __restrict__ out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+
+__global__ __launch_bounds__(256) void hp_read(const uint4* __restrict__ in, uint32_t* sink, int64_t n) {
+    uint32_t acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint4 v = in[i];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (acc == 0x9e3779b9u) sink[0] = acc;   // keeps the loads; the fill pattern never produces it
+}
+
+__global__ __launch_bounds__(256) void hp_write(uint4* __restrict__ out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = make_uint4((uint32_t)i, 1u, 2u, 3u);
+}
+
+constexpr int HP_T = 256, HP_G = 32, HP_LT = HP_T + HP_T / 16 + 1;
+constexpr int64_t HP_PLANE = (int64_t)1 << 24;   // words per plane
+constexpr int HP_PLANES = 10;
+
+// One 256 x 32 tile of plane (lin / ntiles): element (o, t, s) at (o*256 + t)*S + s.
+__global__ __launch_bounds__(512) void hp_colpass(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                   int slog, int rev, int ntiles) {
+    __shared__ int32_t lds[HP_G * HP_LT];
+    const int64_t S = (int64_t)1 << slog;
+    const int lin = rev ? (int)gridDim.x - 1 - (int)blockIdx.x : (int)blockIdx.x;
+    const int tile = lin % ntiles, pl = lin / ntiles;
+    const int64_t l0 = (int64_t)tile * HP_G;
+    const int64_t o0 = l0 >> slog, s0 = l0 & (S - 1);
+    const int64_t base = (int64_t)pl * HP_PLANE + o0 * HP_T * S + s0;
+    const int tid = threadIdx.x, g = tid & (HP_G - 1), j = tid / HP_G;
+    int32_t x[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) x[m] = (int32_t)in[base + g + (int64_t)(j + 16 * m) * S];
+#pragma unroll
+    for (int k = 0; k < 16; k++) lds[g * HP_LT + k * 17 + j] = x[k] + pl;
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 16; m++) x[m] = lds[g * HP_LT + (j * 16 + m) + (j * 16 + m) / 16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) out[base + g + (int64_t)(j * 16 + k) * S] = (uint32_t)x[k];
+}
+
+}  // namespace
+
+// Bytes moved per second (GB/s, 1e9) of probe `kind`, best of `reps` timed
+// runs after one warm-up; negative on a HIP error.
+//   0 copy of `mib` MiB, 1 read of `mib` MiB, 2 write of `mib` MiB (grid x 256 threads)
+//   3 column pass, stride 2^8, forward walk; 4 the same pass walked in reverse;
+//     each timed right after a forward column pass of stride 2^16 over the
+//     same ten planes (its output is their input), 640 MiB read + 640 MiB written
+extern "C" double bwp_hbm_rate(int kind, int64_t mib, int grid, int reps) {
+    if (kind < 0 || kind > 4 || reps < 1 || grid < 1) return -1.0;
+    const bool pass = kind >= 3;
+    const int64_t bytes = pass ? HP_PLANES * HP_PLANE * 4 : (mib << 20);
+    if (bytes <= 0 || bytes > ((int64_t)8 << 30)) return -1.0;
+    void *a = nullptr, *b = nullptr;
+    uint32_t* sink = nullptr;
+    double best = 0.0;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    bool ok = hipMalloc(&a, bytes) == hipSuccess && hipMalloc(&b, bytes) == hipSuccess &&
+              hipMalloc(&sink, 64) == hipSuccess && hipMemset(a, 1, bytes) == hipSuccess &&
+              hipMemset(b, 2, bytes) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+              hipEventCreate(&e1) == hipSuccess && hipEventCreate(&e2) == hipSuccess;
+    const int64_t n16 = bytes / 16;
+    const int ntiles = (int)(HP_PLANE / (HP_T * HP_G));
+    for (int r = 0; ok && r <= reps; r++) {
+        float ms = 0.f;
+        if (pass) {
+            hipLaunchKernelGGL(hp_colpass, dim3(ntiles * HP_PLANES), dim3(512), 0, 0, (const uint32_t*)a, (uint32_t*)b,
+                               16, 0, ntiles);
+            ok = hipEventRecord(e1, 0) == hipSuccess;
+            hipLaunchKernelGGL(hp_colpass, dim3(ntiles * HP_PLANES), dim3(512), 0, 0, (const uint32_t*)b, (uint32_t*)a,
+                               8, kind == 4 ? 1 : 0, ntiles);
+            ok = ok && hipEventRecord(e2, 0) == hipSuccess && hipEventSynchronize(e2) == hipSuccess &&
+                 hipGetLastError() == hipSuccess && hipEventElapsedTime(&ms, e1, e2) == hipSuccess;
+        } else {
+            ok = hipEventRecord(e0, 0) == hipSuccess;
+            if (kind == 0)
+                hipLaunchKernelGGL(hp_copy, dim3(grid), dim3(256), 0, 0, (const uint4*)a, (uint4*)b, n16);
+            else if (kind == 1)
+                hipLaunchKernelGGL(hp_read, dim3(grid), dim3(256), 0, 0, (const uint4*)a, sink, n16);
+            else
+                hipLaunchKernelGGL(hp_write, dim3(grid), dim3(256), 0, 0, (uint4*)b, n16);
+            ok = ok && hipEventRecord(e1, 0) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+                 hipGetLastError() == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+        }
+        const double moved = (kind == 0 || pass) ? 2.0 * (double)bytes : (double)bytes;
+        if (ok && r > 0 && ms > 0.f) {
+            const double gbs = moved / (ms * 1e-3) / 1e9;
+            if (gbs > best) best = gbs;
+        }
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (e2) hipEventDestroy(e2);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    if (sink) hipFree(sink);
+    return ok ? best : -1.0;
+}
