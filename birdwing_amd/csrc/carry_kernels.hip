@@ -192,14 +192,15 @@ __device__ __forceinline__ Wd shift_up(Wd cur, Wd prev, int j, int lane) {
 #define FLAG_INC 2ULL
 
 template <class SRC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SRC::WAVES))) void garner_carry_kernel(CarryParams prm) {
+__global__ __launch_bounds__(64 * BW_CARRY_WAVES) __attribute__((amdgpu_waves_per_eu(SRC::WAVES))) void garner_carry_kernel(CarryParams prm) {
     using Wd = typename SRC::Word;
     constexpr int NW = SRC::NW;
     constexpr int K = BW_CARRY_ROWS;
     constexpr Wd ONES = (Wd)~(Wd)0;
     __shared__ int64_t s_tile;
-    __shared__ uint32_t s_wagg[4];
-    __shared__ uint32_t s_wcin[4];
+    constexpr int NWV = BW_CARRY_WAVES;
+    __shared__ uint32_t s_wagg[NWV];
+    __shared__ uint32_t s_wcin[NWV];
 
     if (threadIdx.x == 0) {
         const int t = atomicAdd(prm.counter, 1);
@@ -236,8 +237,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SRC::WAVES)
     // Every wave hands those to its successor through LDS (slot wave + 1);
     // wave 0 computes them for the tile before (slot 0).  Row 0 is therefore
     // finished last, after a barrier, from its coefficient words kept in cw0.
-    __shared__ Wd s_tw[5][NW][NW];       // [slot][j][lane - (64 - NW)]: word j of the last lanes
-    __shared__ Wd s_thi[5];              // [slot]: high part of the last row's sum at lane 63
+    __shared__ Wd s_tw[NWV + 1][NW][NW];   // [slot][j][lane - (64 - NW)]: word j of the last lanes
+    __shared__ Wd s_thi[NWV + 1];          // [slot]: high part of the last row's sum at lane 63
     auto row_sums = [&](const Wd* cw, const Wd* pw, Wd& lo, Wd& hi) {
         lo = cw[0];
         hi = 0;
@@ -300,7 +301,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SRC::WAVES)
         for (int j = 0; j < NW; j++) pw[j] = cw[j];
         phi = hi;
     }
-    if (wave < 3) publish_tail(wave + 1, pw, phi);
+    if (wave < NWV - 1) publish_tail(wave + 1, pw, phi);
     __syncthreads();
     {
         // row 0 with the tail of the row before it
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SRC::WAVES)
 
     if (threadIdx.x == 0) {
         uint32_t gen = 0, prop = 1;
-        for (int w = 0; w < 4; w++) {
+        for (int w = 0; w < NWV; w++) {
             const uint32_t a = s_wagg[w];
             gen = (a & 1) | ((a >> 1) & gen);
             prop &= (a >> 1);
@@ -385,7 +386,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SRC::WAVES)
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         uint32_t c = cin;
-        for (int w = 0; w < 4; w++) {
+        for (int w = 0; w < NWV; w++) {
             s_wcin[w] = c;
             const uint32_t a = s_wagg[w];
             c = (a & 1) | ((a >> 1) & c);
@@ -434,7 +435,7 @@ hipError_t bw_launch_carry_reset(unsigned long long* flags, int64_t ntiles, int*
 }
 
 hipError_t bw_launch_garner_carry(const CarryParams& p, hipStream_t stream) {
-    dim3 grid((unsigned)p.ntiles), block(256);
+    dim3 grid((unsigned)p.ntiles), block(64 * BW_CARRY_WAVES);
     switch (p.src) {
     case BW_SRC_GARNER_U32: hipLaunchKernelGGL(garner_carry_kernel<SrcGarnerU32>, grid, block, 0, stream, p); break;
     case BW_SRC_GARNER_U64: hipLaunchKernelGGL(garner_carry_kernel<SrcGarnerU64>, grid, block, 0, stream, p); break;
