@@ -630,8 +630,11 @@ def cpu_pool(products, seconds, label):
 def bench_plain(A, B, bits, pset=-1, passes=None, schedule=-1, steps=20, warmup=3, tune=True, workload=None):
     """One product shape, inputs resident in HBM: autotune (unless told not
     to), W warm-up products, then K timed products between synchronizations
-    with per-launch hipEvents on the launch stream.  Returns (fields, plan
-    facts, the product limbs)."""
+    (one hipEvent between products for the latency percentiles), then the
+    same K products again with a hipEvent after every launch for the
+    per-launch times.  The per-launch events stay out of the first run: on a
+    product of a few launches of a few microseconds each they add more time
+    than the launches take.  Returns (fields, plan facts, the product limbs)."""
     import re
     import torch
     import birdwing_amd as bw
@@ -654,7 +657,6 @@ def bench_plain(A, B, bits, pset=-1, passes=None, schedule=-1, steps=20, warmup=
         plan.execute(da, db, dc, stream)
     torch.cuda.synchronize()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
-    plan.set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev[0].record(stream)
@@ -663,6 +665,10 @@ def bench_plain(A, B, bits, pset=-1, passes=None, schedule=-1, steps=20, warmup=
         ev[i + 1].record(stream)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    plan.set_timing(True)
+    for i in range(steps):
+        plan.execute(da, db, dc, stream)
+    torch.cuda.synchronize()
     plan.set_timing(False)
     per_launch = plan.timing()
     plan.status()

@@ -236,44 +236,163 @@ __device__ void inv_dispatch(const F64PrimeDev* spc, int32_t* lds, const Bundle&
 #undef BW_INV
 }
 
-// Transforms, pointwise products and inverse transforms of a bundle: the
-// centered residues of every product's coefficients end in its five residue
-// planes.
+// Transforms, pointwise products and inverse transforms of the bundle's
+// primes [q0, q0 + NB): the centered residues of every product's
+// coefficients under those primes end in their residue planes.
 template <int NB>
-__device__ void bundle_transforms(const FusedParams& prm, const F64PrimeDev* spc, const BwProductJob* jobs,
-                                  int32_t* lds, Bundle& bd) {
+__device__ void chunk_transforms(const FusedParams& prm, const F64PrimeDev* spc, const BwProductJob* jobs,
+                                 int32_t* lds, const Bundle& bd) {
     const int L = bd.L, n = 1 << L;
     const int ns = stage_count(L);
     const int tid = threadIdx.x;
-    for (bd.q0 = 0; bd.q0 < 5; bd.q0 += NB) {
-        for (int idx = tid; idx < NB << L; idx += NT) {
-            const int qi = idx >> L, e = idx & (n - 1);
-            lds[(qi << L) + sw(e)] =
-                (int32_t)prm.tw_stage[(size_t)(bd.q0 + qi) * TW_N + ((size_t)e << (BW_STAGE_LOG - L))].x;
-        }
+    for (int idx = tid; idx < NB << L; idx += NT) {
+        const int qi = idx >> L, e = idx & (n - 1);
+        lds[(qi << L) + sw(e)] =
+            (int32_t)prm.tw_stage[(size_t)(bd.q0 + qi) * TW_N + ((size_t)e << (BW_STAGE_LOG - L))].x;
+    }
+    __syncthreads();
+    for (int s = 0; s < ns; s++) {
+        fwd_dispatch<NB>(prm, spc, jobs, lds, bd, s);
         __syncthreads();
-        for (int s = 0; s < ns; s++) {
-            fwd_dispatch<NB>(prm, spc, jobs, lds, bd, s);
-            __syncthreads();
-        }
-        // pointwise product with n^-1 = p - (p - 1)/n (exact in FP64); A and
-        // B are swizzled alike, so positions pair up directly
-        for (int idx = tid; idx < (bd.K * NB) << L; idx += NT) {
-            const int ta = idx >> L, e = idx & (n - 1);
-            const int k = ta / NB, qi = ta - k * NB;
-            const F64Prime P = spc[bd.q0 + qi].P;
-            const double ni = P.p - P.cm * (1.0 / (double)n);
-            const double nip[2] = {ni, ni * P.pinv};
-            int32_t* a = lds + NB * n + k * (5 + NB) * n + (bd.q0 + qi) * n;
-            const int32_t* b = lds + NB * n + k * (5 + NB) * n + (5 + qi) * n;
-            a[e] = (int32_t)AR::mul_scaled((double)a[e], (double)b[e], nip, P);
-        }
+    }
+    // pointwise product with n^-1 = p - (p - 1)/n (exact in FP64); A and
+    // B are swizzled alike, so positions pair up directly
+    for (int idx = tid; idx < (bd.K * NB) << L; idx += NT) {
+        const int ta = idx >> L, e = idx & (n - 1);
+        const int k = ta / NB, qi = ta - k * NB;
+        const F64Prime P = spc[bd.q0 + qi].P;
+        const double ni = P.p - P.cm * (1.0 / (double)n);
+        const double nip[2] = {ni, ni * P.pinv};
+        int32_t* a = lds + NB * n + k * (5 + NB) * n + (bd.q0 + qi) * n;
+        const int32_t* b = lds + NB * n + k * (5 + NB) * n + (5 + qi) * n;
+        a[e] = (int32_t)AR::mul_scaled((double)a[e], (double)b[e], nip, P);
+    }
+    __syncthreads();
+    for (int s = ns - 1; s >= 0; s--) {
+        inv_dispatch<NB>(spc, lds, bd, s);
         __syncthreads();
-        for (int s = ns - 1; s >= 0; s--) {
-            inv_dispatch<NB>(spc, lds, bd, s);
-            __syncthreads();
+    }
+}
+
+// All five primes of a bundle, NB at a time.
+template <int NB>
+__device__ void bundle_transforms(const FusedParams& prm, const F64PrimeDev* spc, const BwProductJob* jobs,
+                                  int32_t* lds, Bundle& bd) {
+    for (bd.q0 = 0; bd.q0 < 5; bd.q0 += NB) chunk_transforms<NB>(prm, spc, jobs, lds, bd);
+}
+
+// Garner and carry of a bundle whose five residue planes per product are in
+// LDS (product k's at lds + PB n + k (5 + PB) n, position sw(c)): Garner's
+// three 64-bit words of every coefficient, then the carry by rows of 64
+// limbs (product k owns rows [k rp, (k + 1) rp)), result limbs to HBM.
+// Overwrites the planes.  s_row, s_cin: MAX_ROWS words of LDS each.
+template <int PB>
+__device__ void bundle_garner_carry(const FusedParams& prm, const BwProductJob* s_jobs, int32_t* lds,
+                                    const Bundle& bd, uint32_t* s_row, uint32_t* s_cin) {
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int L = bd.L, n = 1 << L;
+    // Garner: coefficient words into registers, then over the planes
+    const int ncoef = bd.K << L;
+    uint64_t cw[CPT][3];
+#pragma unroll
+    for (int r = 0; r < CPT; r++) {
+        const int i = tid + r * NT;
+        if (i < ncoef) {
+            const int k = i >> L, c = i & (n - 1);
+            const int32_t* planes = lds + PB * n + k * (5 + PB) * n;
+            uint32_t rr[5];
+#pragma unroll
+            for (int q = 0; q < 5; q++) rr[q] = (uint32_t)planes[q * n + sw(c)];
+            garner_d64(rr, prm.gd, cw[r]);
         }
     }
+    __syncthreads();
+    // product k's words at W + 3 k n: word j of coefficient c at j n + c
+    uint64_t* W = reinterpret_cast<uint64_t*>(lds);
+#pragma unroll
+    for (int r = 0; r < CPT; r++) {
+        const int i = tid + r * NT;
+        if (i < ncoef) {
+            const int k = i >> L, c = i & (n - 1);
+            uint64_t* Wk = W + (size_t)3 * k * n;
+            Wk[c] = cw[r][0];
+            Wk[n + c] = cw[r][1];
+            Wk[2 * n + c] = cw[r][2];
+        }
+    }
+    __syncthreads();
+    // carry: limb i of product k = t_i + carry_i
+    const int rp = rows_per(L);
+    const int nrows = bd.K * rp;
+    uint64_t tv[CARRY_ROWS], Gm[CARRY_ROWS], Pm[CARRY_ROWS];
+#pragma unroll
+    for (int r = 0; r < CARRY_ROWS; r++) {
+        const int row = wave + r * NWAVES;
+        const int k = row / rp, lr = row - k * rp;
+        const bool live = row < nrows;
+        const uint64_t* Wk = W + (size_t)3 * (live ? k : 0) * n;
+        auto word = [&](int i, int c) -> uint64_t { return (live && i >= 0 && i < n) ? Wk[c * n + i] : 0ull; };
+        const int i = lr * 64 + lane;
+        const uint64_t a0 = word(i, 0), a1 = word(i - 1, 1), a2 = word(i - 2, 2);
+        const uint64_t l1 = a0 + a1, lo = l1 + a2;
+        const uint64_t b0 = word(i - 1, 0), b1 = word(i - 2, 1), b2 = word(i - 3, 2);
+        const uint64_t m1 = b0 + b1, m2 = m1 + b2;
+        const uint64_t hprev = (uint64_t)(m1 < b0) + (uint64_t)(m2 < m1);   // hi(s_{i-1}) <= 2
+        const uint64_t t = lo + hprev;
+        const bool g = t < lo;
+        const bool p = !g && t == ~0ull;
+        tv[r] = t;
+        Gm[r] = __ballot(g);
+        Pm[r] = __ballot(p);
+        if (lane == 0 && live) {
+            const uint64_t X = Gm[r] | Pm[r];
+            const uint64_t sum = X + Gm[r];
+            s_row[row] = (sum < X ? 1u : 0u) | (Pm[r] == ~0ull ? 2u : 0u);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t c = 0;
+        for (int row = 0; row < nrows; row++) {
+            if (row % rp == 0) c = 0;           // a product's first row
+            s_cin[row] = c;
+            const uint32_t a = s_row[row];
+            c = (a & 1u) | ((a >> 1) & c);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < CARRY_ROWS; r++) {
+        const int row = wave + r * NWAVES;
+        if (row < nrows) {
+            const int k = row / rp, lr = row - k * rp;
+            const BwProductJob& J = s_jobs[k];
+            const int64_t nl = J.la + J.lb;
+            const int64_t n_out = J.limb_bits == 32 ? (nl + 1) >> 1 : nl;
+            const bool half_last = J.limb_bits == 32 && (nl & 1);
+            const uint64_t X = Gm[r] | Pm[r];
+            const uint64_t s2 = X + Gm[r] + (uint64_t)s_cin[row];
+            const uint64_t carries = s2 ^ Pm[r];
+            const int i = lr * 64 + lane;
+            const uint64_t limb = tv[r] + ((carries >> lane) & 1ull);
+            uint64_t* out = reinterpret_cast<uint64_t*>(prm.C + J.c_off);
+            if (i < n_out) {
+                if (half_last && i == n_out - 1) reinterpret_cast<uint32_t*>(out + i)[0] = (uint32_t)limb;
+                else out[i] = limb;
+            }
+        }
+    }
+}
+
+// The five primes' constants into LDS (every thread takes part; a barrier
+// must follow before they are read).
+__device__ void load_consts(const FusedParams& prm, F64PrimeDev* spc) {
+    constexpr int WORDS = (int)(sizeof(F64PrimeDev) / 8);
+    static_assert(sizeof(F64PrimeDev) % 8 == 0, "8-byte words");
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(prm.consts);
+    uint64_t* dst = reinterpret_cast<uint64_t*>(spc);
+    for (int i = threadIdx.x; i < 5 * WORDS; i += NT) dst[i] = src[i];
 }
 
 }  // namespace
@@ -289,14 +408,7 @@ __global__ __launch_bounds__(NT) void product_fused_kernel(FusedParams prm) {
     // serve one lane at a time
     __shared__ F64PrimeDev spc[5];
     const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
-    {
-        constexpr int WORDS = (int)(sizeof(F64PrimeDev) / 8);
-        static_assert(sizeof(F64PrimeDev) % 8 == 0, "8-byte words");
-        const uint64_t* src = reinterpret_cast<const uint64_t*>(prm.consts);
-        uint64_t* dst = reinterpret_cast<uint64_t*>(spc);
-        for (int i = tid; i < 5 * WORDS; i += NT) dst[i] = src[i];
-    }
+    load_consts(prm, spc);
     for (;;) {
         if (tid == 0) {
             // every workgroup takes exactly one id past the last bundle; the
@@ -328,98 +440,7 @@ __global__ __launch_bounds__(NT) void product_fused_kernel(FusedParams prm) {
             continue;
         }
         bundle_transforms<NB>(prm, spc, s_jobs, lds, bd);
-        // Garner: coefficient words into registers, then over the planes
-        const int ncoef = bd.K << L;
-        uint64_t cw[CPT][3];
-#pragma unroll
-        for (int r = 0; r < CPT; r++) {
-            const int i = tid + r * NT;
-            if (i < ncoef) {
-                const int k = i >> L, c = i & (n - 1);
-                const int32_t* planes = lds + NB * n + k * (5 + NB) * n;
-                uint32_t rr[5];
-#pragma unroll
-                for (int q = 0; q < 5; q++) rr[q] = (uint32_t)planes[q * n + sw(c)];
-                garner_d64(rr, prm.gd, cw[r]);
-            }
-        }
-        __syncthreads();
-        // product k's words at W + 3 k n: word j of coefficient c at j n + c
-        uint64_t* W = reinterpret_cast<uint64_t*>(lds);
-#pragma unroll
-        for (int r = 0; r < CPT; r++) {
-            const int i = tid + r * NT;
-            if (i < ncoef) {
-                const int k = i >> L, c = i & (n - 1);
-                uint64_t* Wk = W + (size_t)3 * k * n;
-                Wk[c] = cw[r][0];
-                Wk[n + c] = cw[r][1];
-                Wk[2 * n + c] = cw[r][2];
-            }
-        }
-        __syncthreads();
-        // carry: limb i of product k = t_i + carry_i; rows of 64 limbs, product
-        // k owns rows [k rp, (k + 1) rp)
-        const int rp = rows_per(L);
-        const int nrows = bd.K * rp;
-        uint64_t tv[CARRY_ROWS], Gm[CARRY_ROWS], Pm[CARRY_ROWS];
-#pragma unroll
-        for (int r = 0; r < CARRY_ROWS; r++) {
-            const int row = wave + r * NWAVES;
-            const int k = row / rp, lr = row - k * rp;
-            const bool live = row < nrows;
-            const uint64_t* Wk = W + (size_t)3 * (live ? k : 0) * n;
-            auto word = [&](int i, int c) -> uint64_t { return (live && i >= 0 && i < n) ? Wk[c * n + i] : 0ull; };
-            const int i = lr * 64 + lane;
-            const uint64_t a0 = word(i, 0), a1 = word(i - 1, 1), a2 = word(i - 2, 2);
-            const uint64_t l1 = a0 + a1, lo = l1 + a2;
-            const uint64_t b0 = word(i - 1, 0), b1 = word(i - 2, 1), b2 = word(i - 3, 2);
-            const uint64_t m1 = b0 + b1, m2 = m1 + b2;
-            const uint64_t hprev = (uint64_t)(m1 < b0) + (uint64_t)(m2 < m1);   // hi(s_{i-1}) <= 2
-            const uint64_t t = lo + hprev;
-            const bool g = t < lo;
-            const bool p = !g && t == ~0ull;
-            tv[r] = t;
-            Gm[r] = __ballot(g);
-            Pm[r] = __ballot(p);
-            if (lane == 0 && live) {
-                const uint64_t X = Gm[r] | Pm[r];
-                const uint64_t sum = X + Gm[r];
-                s_row[row] = (sum < X ? 1u : 0u) | (Pm[r] == ~0ull ? 2u : 0u);
-            }
-        }
-        __syncthreads();
-        if (tid == 0) {
-            uint32_t c = 0;
-            for (int row = 0; row < nrows; row++) {
-                if (row % rp == 0) c = 0;           // a product's first row
-                s_cin[row] = c;
-                const uint32_t a = s_row[row];
-                c = (a & 1u) | ((a >> 1) & c);
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < CARRY_ROWS; r++) {
-            const int row = wave + r * NWAVES;
-            if (row < nrows) {
-                const int k = row / rp, lr = row - k * rp;
-                const BwProductJob& J = s_jobs[k];
-                const int64_t nl = J.la + J.lb;
-                const int64_t n_out = J.limb_bits == 32 ? (nl + 1) >> 1 : nl;
-                const bool half_last = J.limb_bits == 32 && (nl & 1);
-                const uint64_t X = Gm[r] | Pm[r];
-                const uint64_t s2 = X + Gm[r] + (uint64_t)s_cin[row];
-                const uint64_t carries = s2 ^ Pm[r];
-                const int i = lr * 64 + lane;
-                const uint64_t limb = tv[r] + ((carries >> lane) & 1ull);
-                uint64_t* out = reinterpret_cast<uint64_t*>(prm.C + J.c_off);
-                if (i < n_out) {
-                    if (half_last && i == n_out - 1) reinterpret_cast<uint32_t*>(out + i)[0] = (uint32_t)limb;
-                    else out[i] = limb;
-                }
-            }
-        }
+        bundle_garner_carry<NB>(prm, s_jobs, lds, bd, s_row, s_cin);
         __syncthreads();
     }
 }
