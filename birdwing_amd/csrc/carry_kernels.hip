@@ -191,21 +191,25 @@ __device__ __forceinline__ Wd shift_up(Wd cur, Wd prev, int j, int lane) {
 #define FLAG_AGG 1ULL
 #define FLAG_INC 2ULL
 
-template <class SRC>
-__global__ __launch_bounds__(64 * BW_CARRY_WAVES) __attribute__((amdgpu_waves_per_eu(SRC::WAVES))) void garner_carry_kernel(CarryParams prm) {
+// K rows of 64 limbs per wave, NWV waves: a tile of 64 K NWV limbs.
+template <class SRC, int K = BW_CARRY_ROWS, int NWV = BW_CARRY_WAVES>
+__global__ __launch_bounds__(64 * NWV) __attribute__((amdgpu_waves_per_eu(SRC::WAVES))) void garner_carry_kernel(CarryParams prm) {
     using Wd = typename SRC::Word;
     constexpr int NW = SRC::NW;
-    constexpr int K = BW_CARRY_ROWS;
+    constexpr int64_t TILE = (int64_t)64 * K * NWV;
     constexpr Wd ONES = (Wd)~(Wd)0;
     __shared__ int64_t s_tile;
-    constexpr int NWV = BW_CARRY_WAVES;
     __shared__ uint32_t s_wagg[NWV];
     __shared__ uint32_t s_wcin[NWV];
 
     if (threadIdx.x == 0) {
-        const int t = atomicAdd(prm.counter, 1);
-        if ((int64_t)t == prm.ntiles - 1) atomicExch(prm.counter, 0);   // every id is taken: reset for the next launch
-        s_tile = t;
+        if (prm.ntiles == 1) {
+            s_tile = 0;                        // one tile: no id to take, nothing to chain
+        } else {
+            const int t = atomicAdd(prm.counter, 1);
+            if ((int64_t)t == prm.ntiles - 1) atomicExch(prm.counter, 0);   // every id is taken: reset for the next launch
+            s_tile = t;
+        }
     }
     __syncthreads();
     const int64_t tile = s_tile;
@@ -227,7 +231,7 @@ __global__ __launch_bounds__(64 * BW_CARRY_WAVES) __attribute__((amdgpu_waves_pe
         ltile = tile - J.tile0;
         half_last = J.half_last != 0;
     }
-    const int64_t wbase = ltile * BW_CARRY_TILE + (int64_t)wave * 64 * K;
+    const int64_t wbase = ltile * TILE + (int64_t)wave * 64 * K;
 
     Wd tval[K];
     bool tg[K], tp[K];
@@ -434,15 +438,30 @@ hipError_t bw_launch_carry_reset(unsigned long long* flags, int64_t ntiles, int*
     return hipGetLastError();
 }
 
+// A single product of at most 4096 output limbs is one tile whatever its
+// shape; there the latency of the one workgroup decides, so it runs sixteen
+// waves of two or four rows each (the Garner work of the tile spread over
+// twice the waves of the default shape).
+template <class SRC>
+static void launch_carry(const CarryParams& p, hipStream_t stream) {
+    if (!p.jobs && p.ntiles == 1 && p.n_out <= 64 * 2 * 16)
+        hipLaunchKernelGGL((garner_carry_kernel<SRC, 2, 16>), dim3(1), dim3(1024), 0, stream, p);
+    else if (!p.jobs && p.ntiles == 1 && p.n_out <= 64 * 4 * 16)
+        hipLaunchKernelGGL((garner_carry_kernel<SRC, 4, 16>), dim3(1), dim3(1024), 0, stream, p);
+    else
+        hipLaunchKernelGGL(garner_carry_kernel<SRC>, dim3((unsigned)p.ntiles), dim3(64 * BW_CARRY_WAVES), 0, stream,
+                           p);
+}
+
 hipError_t bw_launch_garner_carry(const CarryParams& p, hipStream_t stream) {
     dim3 grid((unsigned)p.ntiles), block(64 * BW_CARRY_WAVES);
     switch (p.src) {
-    case BW_SRC_GARNER_U32: hipLaunchKernelGGL(garner_carry_kernel<SrcGarnerU32>, grid, block, 0, stream, p); break;
-    case BW_SRC_GARNER_U64: hipLaunchKernelGGL(garner_carry_kernel<SrcGarnerU64>, grid, block, 0, stream, p); break;
+    case BW_SRC_GARNER_U32: launch_carry<SrcGarnerU32>(p, stream); break;
+    case BW_SRC_GARNER_U64: launch_carry<SrcGarnerU64>(p, stream); break;
     case BW_SRC_HILO128: hipLaunchKernelGGL(garner_carry_kernel<SrcHiLo128>, grid, block, 0, stream, p); break;
     case BW_SRC_HILO128_L64: hipLaunchKernelGGL(garner_carry_kernel<SrcHiLo128L64>, grid, block, 0, stream, p); break;
     case BW_SRC_U160: hipLaunchKernelGGL(garner_carry_kernel<SrcU160>, grid, block, 0, stream, p); break;
-    case BW_SRC_GARNER_D64: hipLaunchKernelGGL(garner_carry_kernel<SrcGarnerD64>, grid, block, 0, stream, p); break;
+    case BW_SRC_GARNER_D64: launch_carry<SrcGarnerD64>(p, stream); break;
     case BW_SRC_GARNER_U64X2: hipLaunchKernelGGL(garner_carry_kernel<SrcGarnerU64x2>, grid, block, 0, stream, p); break;
     default: return hipErrorInvalidValue;
     }
