@@ -115,7 +115,9 @@ def hbm_rates(mib=640, reps=10):
     write of `mib` MiB at three grid sizes, and a 256-point column pass of
     stride 2^8 over ten 2^24-word planes (the memory pattern of the middle
     forward pass, no arithmetic) walked forward and in reverse right after
-    a pass of stride 2^16 over the same planes."""
+    a pass of stride 2^16 over the same planes; the stride-2^16 pass timed
+    after one of stride 2^8; both over planes padded with one 128-byte line
+    after every 2^16 words."""
     import torch
     torch.cuda.init()
     out = {}
@@ -125,7 +127,9 @@ def hbm_rates(mib=640, reps=10):
             if r < 0:
                 raise RuntimeError("hbm probe %s failed" % name)
             out["%s_grid%d" % (name, grid)] = r
-    for kind, name in ((3, "colpass_s256_forward_walk"), (4, "colpass_s256_reverse_walk")):
+    for kind, name in ((3, "colpass_s256_forward_walk"), (4, "colpass_s256_reverse_walk"),
+                       (5, "colpass_s65536_forward_walk"), (6, "colpass_s65536_padded_planes"),
+                       (7, "colpass_s256_padded_planes")):
         r = float(lib().bwp_hbm_rate(kind, 0, 1, reps))
         if r < 0:
             raise RuntimeError("hbm probe %s failed" % name)

@@ -34,27 +34,34 @@ constexpr int HP_T = 256, HP_G = 32, HP_LT = HP_T + HP_T / 16 + 1;
 constexpr int64_t HP_PLANE = (int64_t)1 << 24;   // words per plane
 constexpr int HP_PLANES = 10;
 
-// One 256 x 32 tile of plane (lin / ntiles): element (o, t, s) at (o*256 + t)*S + s.
+constexpr int HP_PAD = 32;                       // words after every 2^16-word block (padded kinds)
+constexpr int64_t HP_PSTRIDE = HP_PLANE + 256 * HP_PAD;
+
+// One 256 x 32 tile of plane (lin / ntiles): element e = (o*256 + t)*S + s
+// at e + (e >> 16) * pad of its plane (planes 2^24 words apart, or
+// HP_PSTRIDE when padded).
 __global__ __launch_bounds__(512) void hp_colpass(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                   int slog, int rev, int ntiles) {
+                                                   int slog, int rev, int ntiles, int pad) {
     __shared__ int32_t lds[HP_G * HP_LT];
     const int64_t S = (int64_t)1 << slog;
     const int lin = rev ? (int)gridDim.x - 1 - (int)blockIdx.x : (int)blockIdx.x;
     const int tile = lin % ntiles, pl = lin / ntiles;
     const int64_t l0 = (int64_t)tile * HP_G;
     const int64_t o0 = l0 >> slog, s0 = l0 & (S - 1);
-    const int64_t base = (int64_t)pl * HP_PLANE + o0 * HP_T * S + s0;
+    const int64_t e0 = o0 * HP_T * S + s0;
     const int tid = threadIdx.x, g = tid & (HP_G - 1), j = tid / HP_G;
+    const int64_t ps = pad ? HP_PSTRIDE : HP_PLANE;
+    auto at = [&](int64_t e) { return (int64_t)pl * ps + e + (e >> 16) * pad; };
     int32_t x[16];
 #pragma unroll
-    for (int m = 0; m < 16; m++) x[m] = (int32_t)in[base + g + (int64_t)(j + 16 * m) * S];
+    for (int m = 0; m < 16; m++) x[m] = (int32_t)in[at(e0 + g + (int64_t)(j + 16 * m) * S)];
 #pragma unroll
     for (int k = 0; k < 16; k++) lds[g * HP_LT + k * 17 + j] = x[k] + pl;
     __syncthreads();
 #pragma unroll
     for (int m = 0; m < 16; m++) x[m] = lds[g * HP_LT + (j * 16 + m) + (j * 16 + m) / 16];
 #pragma unroll
-    for (int k = 0; k < 16; k++) out[base + g + (int64_t)(j * 16 + k) * S] = (uint32_t)x[k];
+    for (int k = 0; k < 16; k++) out[at(e0 + g + (int64_t)(j * 16 + k) * S)] = (uint32_t)x[k];
 }
 
 }  // namespace
@@ -65,10 +72,13 @@ __global__ __launch_bounds__(512) void hp_colpass(const uint32_t* __restrict__ i
 //   3 column pass, stride 2^8, forward walk; 4 the same pass walked in reverse;
 //     each timed right after a forward column pass of stride 2^16 over the
 //     same ten planes (its output is their input), 640 MiB read + 640 MiB written
+//   5 column pass, stride 2^16, forward walk, timed right after one of stride 2^8
+//   6, 7: kinds 5 and 3 over planes padded with 32 words after every 2^16-word
+//     block (rows of the stride-2^16 pass 2^16 + 32 words apart)
 extern "C" double bwp_hbm_rate(int kind, int64_t mib, int grid, int reps) {
-    if (kind < 0 || kind > 4 || reps < 1 || grid < 1) return -1.0;
+    if (kind < 0 || kind > 7 || reps < 1 || grid < 1) return -1.0;
     const bool pass = kind >= 3;
-    const int64_t bytes = pass ? HP_PLANES * HP_PLANE * 4 : (mib << 20);
+    const int64_t bytes = pass ? HP_PLANES * HP_PSTRIDE * 4 : (mib << 20);
     if (bytes <= 0 || bytes > ((int64_t)8 << 30)) return -1.0;
     void *a = nullptr, *b = nullptr;
     uint32_t* sink = nullptr;
@@ -83,11 +93,13 @@ extern "C" double bwp_hbm_rate(int kind, int64_t mib, int grid, int reps) {
     for (int r = 0; ok && r <= reps; r++) {
         float ms = 0.f;
         if (pass) {
+            const int timed_log = (kind == 5 || kind == 6) ? 16 : 8;
+            const int pad = (kind == 6 || kind == 7) ? HP_PAD : 0;
             hipLaunchKernelGGL(hp_colpass, dim3(ntiles * HP_PLANES), dim3(512), 0, 0, (const uint32_t*)a, (uint32_t*)b,
-                               16, 0, ntiles);
+                               24 - timed_log, 0, ntiles, pad);
             ok = hipEventRecord(e1, 0) == hipSuccess;
             hipLaunchKernelGGL(hp_colpass, dim3(ntiles * HP_PLANES), dim3(512), 0, 0, (const uint32_t*)b, (uint32_t*)a,
-                               8, kind == 4 ? 1 : 0, ntiles);
+                               timed_log, kind == 4 ? 1 : 0, ntiles, pad);
             ok = ok && hipEventRecord(e2, 0) == hipSuccess && hipEventSynchronize(e2) == hipSuccess &&
                  hipGetLastError() == hipSuccess && hipEventElapsedTime(&ms, e1, e2) == hipSuccess;
         } else {
@@ -101,17 +113,17 @@ extern "C" double bwp_hbm_rate(int kind, int64_t mib, int grid, int reps) {
             ok = ok && hipEventRecord(e1, 0) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
                  hipGetLastError() == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
         }
-        const double moved = (kind == 0 || pass) ? 2.0 * (double)bytes : (double)bytes;
+        const double moved = pass ? 2.0 * (double)(HP_PLANES * HP_PLANE * 4) : kind == 0 ? 2.0 * (double)bytes : (double)bytes;
         if (ok && r > 0 && ms > 0.f) {
             const double gbs = moved / (ms * 1e-3) / 1e9;
             if (gbs > best) best = gbs;
         }
     }
-    if (e0) hipEventDestroy(e0);
-    if (e1) hipEventDestroy(e1);
-    if (e2) hipEventDestroy(e2);
-    if (a) hipFree(a);
-    if (b) hipFree(b);
-    if (sink) hipFree(sink);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e2) (void)hipEventDestroy(e2);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (sink) (void)hipFree(sink);
     return ok ? best : -1.0;
 }
