@@ -523,10 +523,12 @@ def test_config_plain_large(gpu, oracle, bits):
 
 
 @pytest.mark.parametrize("bits", [32, 64])
-@pytest.mark.parametrize("LA,LB", [(1, 1), (3, 5), (1000, 1003), (4097, 4095), (1 << 15, 1 << 15),
-                                   (70000, 1000), (1 << 18, (1 << 18) - 1)])
+@pytest.mark.parametrize("LA,LB", [(1, 1), (3, 5), (1000, 1003), (1500, 2500), (2048, 2048), (4097, 4095),
+                                   (1 << 15, 1 << 15), (70000, 1000), (1 << 18, (1 << 18) - 1)])
 def test_three_prime_products_vs_gmp(gpu, oracle, bits, LA, LB):
-    """The reference pipelines' three-prime sets (chosen explicitly) against GMP."""
+    """The reference pipelines' three-prime sets (chosen explicitly) against GMP
+    (up to 2048 output limbs the carry runs one tile of sixteen waves of two
+    rows, up to 4096 one of four rows, beyond that tiles of the default shape)."""
     A = oracle.uniform_limbs(LA, LA * 3 + 7, bits)
     B = oracle.uniform_limbs(LB, LB * 5 + 9, bits)
     s = gpu.default_prime_set(bits, gpu.padded_ntt_size(LA, LB))
