@@ -23,19 +23,24 @@ __device__ __forceinline__ void garner_d64(const uint32_t* r, const GarnerD64Dev
         kd[j] = t < 0.0 ? t + g.fp[j].p : t;
         k[j] = (uint32_t)kd[j];
     }
+    // Before step j, x = k_{j+1} + p_{j+1} (k_{j+2} + ...) < p_{j+1} ... p_4
+    // < 2^(31 (4 - j)) fills at most 4 - j words, and k_j + p_j x at most one
+    // more: the step multiplies only those words (ten multiply-adds, not 20).
     uint32_t x[5] = {k[4], 0u, 0u, 0u, 0u};
 #pragma unroll
     for (int j = 3; j >= 0; j--) {
-        // the prime as a wave-uniform 32-bit scalar: each step is one
+        // the prime as a wave-uniform 32-bit scalar: each word is one
         // 32 x 32 + 64-bit multiply-add (v_mad_u64_u32)
         const uint32_t pj = (uint32_t)__builtin_amdgcn_readfirstlane((int)g.p[j]);
+        const int nw = 4 - j;
         uint64_t c = k[j];
 #pragma unroll
-        for (int i = 0; i < 5; i++) {
+        for (int i = 0; i < nw; i++) {
             c += (uint64_t)x[i] * pj;
             x[i] = (uint32_t)c;
             c >>= 32;
         }
+        x[nw] = (uint32_t)c;
     }
     w[0] = x[0] | ((uint64_t)x[1] << 32);
     w[1] = x[2] | ((uint64_t)x[3] << 32);
