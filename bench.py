@@ -699,7 +699,8 @@ def bench_plain(A, B, bits, pset=-1, passes=None, schedule=-1, steps=20, warmup=
         "config": {"workload": workload, "mode": "plain", "limbs_a": LA, "limbs_b": LB, "limb_bits": bits,
                    "prime_set": plan.prime_set, "primes": plan.primes, "digit_bits": plan.digit_bits,
                    "N": plan.N, "N_exec": plan.N_exec, "passes": plan.passes,
-                   "schedule": ["pass-major", "prime-major", "hybrid"][sched], "autotuned": tuned},
+                   "schedule": ["pass-major", "prime-major", "hybrid"][sched],
+                   "tile_shift": int(plan.info.tile_shift), "autotuned": tuned},
         "kernels": [{"name": k, "launches": v[2], "ms_per_product": v[0], "alg_bytes": v[1],
                      "alg_GBps": v[1] / (v[0] * 1e-3) / 1e9} for k, v in agg.items()],
         "roofline": {"bound": "hbm", "kernel": dom_name, "launches_per_product": dom_n,

@@ -48,7 +48,7 @@ class PlanInfo(C.Structure):
                 ("primes", C.c_uint64 * 5), ("L_A", C.c_uint64), ("L_B", C.c_uint64),
                 ("N", C.c_uint64), ("N_exec", C.c_uint64), ("npass", C.c_int),
                 ("pass_log", C.c_int * 4), ("nlaunch", C.c_int), ("device_bytes", C.c_uint64),
-                ("digit_bits", C.c_int), ("schedule", C.c_int)]
+                ("digit_bits", C.c_int), ("schedule", C.c_int), ("tile_shift", C.c_int)]
 
 
 class BatchInfo(C.Structure):

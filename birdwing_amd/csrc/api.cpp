@@ -315,6 +315,7 @@ extern "C" int bw_plan_get_info(const bw_plan* p, bw_plan_info* info) {
     info->device_bytes = p->dev_bytes;
     info->digit_bits = p->ingress == 2 ? 64 : p->limb_bits;
     info->schedule = p->schedule;
+    info->tile_shift = p->gshift;
     return BW_OK;
 }
 

@@ -84,12 +84,13 @@ typedef struct bw_plan_info {
     uint64_t device_bytes; /* device memory held by the plan */
     int digit_bits;        /* 64 for the five-prime 64-bit-digit set, else limb_bits */
     int schedule;          /* launch order, BW_SCHED_* */
+    int tile_shift;        /* tile cap: lines per workgroup at most the kernel's maximum >> tile_shift */
 } bw_plan_info;
 
 /* ---- library ---------------------------------------------------------- */
 /* Layout version of the structs in this header; a caller built against one
  * version must not run against a library reporting another. */
-#define BW_ABI_VERSION 3
+#define BW_ABI_VERSION 4
 int bw_abi_version(void);
 const char* bw_last_error(void);
 const char* bw_version(void);
