@@ -424,8 +424,11 @@ def test_plan_autotune_keeps_product(gpu, oracle, bits, L):
     plan.execute(da, db, c)
     want, _ = oracle.gmp_mul(A, B, bits)
     assert np.array_equal(d.from_dev(c, len(A) + len(B)), want), plan.passes
-    # a new plan of the same shape starts from the tuned split
-    assert gpu.Plan(bits, len(A), len(B)).passes == plan.passes
+    # the tuned tile cap is one of the candidates (full, half or quarter width)
+    assert 0 <= plan.info.tile_shift <= 2
+    # a new plan of the same shape starts from the tuned split and tile cap
+    again = gpu.Plan(bits, len(A), len(B))
+    assert again.passes == plan.passes and again.info.tile_shift == plan.info.tile_shift
 
 
 @pytest.mark.parametrize("bits", [32, 64])
