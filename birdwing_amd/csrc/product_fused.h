@@ -16,9 +16,9 @@
 
 #define BW_FUSED_MAX_LOG 11        // largest transform: 2^11 64-bit digits
 // Default limit of batches: the fused launch is faster than stacked
-// transforms up to 2^7 digits (1.8x on MI355X), level at 2^8, slower at 2^9
-// and 2^10 (whose single register-staged pass of radix-32 codelets does
-// fewer LDS round trips per butterfly).
+// transforms up to 2^8 digits (1.2x at 2^8, 2.3x at 2^7, 3-5x below on
+// MI355X), slower at 2^9 and 2^10 (whose single register-staged pass of
+// radix-32 codelets does fewer LDS round trips per butterfly), level at 2^11.
 #define BW_FUSED_DEFAULT_LOG 8
 #define BW_FUSED_THREADS 1024
 #define BW_FUSED_MAX_BUNDLE 16
