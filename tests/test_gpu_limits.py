@@ -106,6 +106,27 @@ def test_three_prime_full_size_all_ones(gpu, oracle, bits):
     assert np.array_equal(got, _square_of_ones(L, bits, bits * L))
 
 
+@pytest.mark.parametrize("bits,la,lb", [(32, 1, 1 << 25), (32, 1 << 25, 3), (64, 1 << 24, 1),
+                                        (64, 2, (1 << 24) - 1)])
+def test_lopsided_full_size_all_ones(gpu, oracle, bits, la, lb):
+    """The most unbalanced shapes at the automatic set's largest operand (a
+    few limbs times 2^25 32-bit or 2^24 64-bit limbs, all ones: every
+    coefficient at its largest for these lengths, carries the full length),
+    vs GMP and the closed form (2^a - 1)(2^b - 1)."""
+    A = _ones(la, bits)
+    B = _ones(lb, bits)
+    print("lopsided all-ones %d x %d u%d" % (la, lb, bits), flush=True)
+    got, (pset, k) = _run(gpu, A, B, bits)
+    assert pset == gpu.SET_D64 and k == 5
+    g, _ = oracle.gmp_mul(A, B, bits)
+    assert np.array_equal(got, g)
+    a, b = bits * la, bits * lb
+    v = (1 << (a + b)) - (1 << a) - (1 << b) + 1
+    dt = np.uint64 if bits == 64 else np.uint32
+    want = np.frombuffer(v.to_bytes((la + lb) * bits // 8, "little"), dtype=dt)
+    assert np.array_equal(got, want)
+
+
 # ---- batches and serving with large products ---------------------------------
 def _value_mod(limbs, q, block=1 << 14):
     """Value mod q (q < 2^31) of little-endian limbs (32- or 64-bit), by
