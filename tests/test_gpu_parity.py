@@ -432,6 +432,19 @@ def test_plan_autotune_keeps_product(gpu, oracle, bits, L):
 
 
 @pytest.mark.parametrize("bits", [32, 64])
+def test_random_shapes_vs_gmp(gpu, oracle, bits):
+    """Forty seeded shapes, limb counts log-uniform in [1, 2^18] per operand
+    (odd, unbalanced, every transform length from 2^1 to 2^18 and the splits
+    the plans pick for them), under the automatic set, each against GMP."""
+    rng = np.random.default_rng(20260 + bits)
+    for i in range(40):
+        la, lb = (int(2.0 ** rng.uniform(0, 18)) for _ in range(2))
+        A = oracle.uniform_limbs(la, 1000 + 2 * i, bits)
+        B = oracle.uniform_limbs(lb, 1001 + 2 * i, bits)
+        _product_check(gpu, oracle, A, B, bits)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
 @pytest.mark.parametrize("LA,LB", [(1, 1), (1, 2), (2, 3), (3, 5), (7, 7), (2047, 2049), (4096, 4096),
                                    (10001, 9999), (1 << 16, (1 << 16) + 1), (100000, 3), (1 << 18, 1 << 18)])
 def test_d64_products_vs_gmp(gpu, oracle, bits, LA, LB):
